@@ -1,0 +1,135 @@
+/*
+ * vikshobha: MI355X (gfx950) implementation of the 2D Gross-Pitaevskii
+ * pseudospectral step of SourangshuGhosh/QuantumVortices ("VIKSHOBHA").
+ *
+ * C ABI.  Plain pointers and sizes; complex numbers are interleaved
+ * (re, im) doubles.  Host arrays use the reference's Fortran views:
+ *   psi(i1=x, i2=y), x fastest            -> psi_xy[2*(y*N + x) + {0,1}]
+ *   kpsi(i2=ky, i1=kx), ky fastest        -> kpsi_t[2*(kx*N + ky) + {0,1}]
+ *     (FFTW_MPI_TRANSPOSED_OUT layout, src/mod_fft.f90:143-146,307-311)
+ *
+ * Return values: 0 = success, < 0 = error (see vk_last_error(), which is
+ * thread-local).  One handle per host thread; each handle owns its device
+ * buffers and one HIP stream; calls are synchronous unless named _async.
+ * Every compute entry point requires a ROCm device; there is no CPU path.
+ */
+#ifndef VIKSHOBHA_H
+#define VIKSHOBHA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VK_OK 0
+#define VK_ERR_ARG -1
+#define VK_ERR_HIP -2
+#define VK_ERR_STATE -3
+#define VK_ERR_UNSUPPORTED -4
+
+/* set_psi flags */
+#define VK_SYNC_DEALIAS 1  /* after kpsi = FFT(psi): dealias kpsi and
+                              psi = IFFT(kpsi), as restart_from_argle_data +
+                              ic_otoc_gpe do (src/initial_config.f90:759,
+                              :316-343) */
+
+typedef struct vk_gpe vk_gpe;
+
+/* Run parameters (the reference's module globals after src/driver.f90:360-398
+ * and src/mod_grid.f90:63-101). */
+typedef struct {
+    int N;              /* Nx = Ny, power of two, 32..4096 */
+    double dt;          /* dt_fix */
+    double alpha;       /* alpha */
+    double g;           /* gstrength */
+    double omega;       /* omega_chem */
+    double disp;        /* disp_factor */
+    double xi;          /* xi (already multiplied by dx) */
+    /* particles (GPPART): number, ppotV0 (already x g), ppotd (already x xi),
+     * gp_mass_part, and 1 to dealias rho-hat in the particle force with
+     * k1^2+k2^2 (the reference's intent; src/gpe_particle.f90:275 tests an
+     * uninitialised ksqr) */
+    int np;
+    double V0, d, mass;
+    int force_dealias;
+} vk_gpe_params;
+
+/* Grid constants and derived GPE constants in the reference's order of
+ * operations: src/mod_grid.f90:63-101 (dx, kalias, nshell) and
+ * src/driver.f90:360-376 (xi*=dx, alpha, gstrength, omega_chem). */
+int vk_gpe_params_init(vk_gpe_params* p, int N, double dt, double xi_in,
+                       double csound, double rho_avg, double disp);
+int vk_nshell(int N);               /* int(1.414*Nx/2.0d0)+1 */
+
+/* Replaces create_state_variables + initialize_fft (src/mod_fft.f90:122-198,
+ * :288-428; FFTW plans + fftw_alloc_complex) and global_array
+ * (src/pspectral.f90:1-35). */
+int vk_gpe_create(vk_gpe** h, const vk_gpe_params* p);
+void vk_gpe_destroy(vk_gpe* h);    /* clear_fft + free_state_variables */
+
+/* Field upload/download.  set_psi: kpsi = FFT(psi) (src/initial_config.f90:759)
+ * and, with VK_SYNC_DEALIAS, the dealias + psi=IFFT(kpsi) sync of
+ * ic_otoc_gpe.  get_psi: either pointer may be NULL. */
+int vk_gpe_set_psi(vk_gpe* h, const double* psi_xy, int flags);
+int vk_gpe_set_kpsi(vk_gpe* h, const double* kpsi_t);
+int vk_gpe_get_psi(vk_gpe* h, double* psi_xy, double* kpsi_t);
+
+/* nsteps x gpe_rk4 (src/gpe.f90:310-494 with gpe_rhs :498-671; with
+ * particles also gp_particle_motion_rk4, src/gpe_particle.f90:429-531). */
+int vk_gpe_rk4_steps(vk_gpe* h, int nsteps);
+int vk_gpe_rk4_steps_async(vk_gpe* h, int nsteps);
+
+/* nsteps x ARGLE(time) (src/gpe.f90:5-224, plain variant).  residual (may be
+ * NULL) receives ||kpsi_new-kpsi_old||_2/(dt*||kpsi_old||_2) of the last
+ * step; the reference computes no residual. */
+int vk_gpe_argle_steps(vk_gpe* h, int nsteps, double* residual);
+
+/* diagnostics_gpe energies and spectra (src/diagnostics.f90:54-426):
+ * scal[8] = {egrad, echem, eintr, etot, norm, momx, momy, n0=spec(0,6)};
+ * spec = spec_gp(0:nshell,1:6) in Fortran order (may be NULL);
+ * pscal[5] = {eppot, eobj_kin, esrep, momobj_x, momobj_y} (particles; may be
+ * NULL). */
+int vk_gpe_diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal);
+
+/* Particles: particle_potential (src/gpe_particle.f90:3-90) runs inside
+ * particles_set; pos/vel/fext are (2,np) Fortran arrays (x,y per particle).
+ * particles_get: frc receives the forces of the last diagnostics call
+ * (gp_frc_obj, src/diagnostics.f90:528-533).  Any pointer may be NULL. */
+int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel,
+                         const double* fext);
+int vk_gpe_particles_get(vk_gpe* h, double* pos, double* vel, double* frc);
+
+/* Conformance hook mirroring fft(plan_name, direction)
+ * (src/mod_fft.f90:471-542): dir = -1: in = x-space (Nx,Ny) -> out = k-space
+ * (Ny,Nx) transposed, unnormalised; dir = +1: inverse times 1/(Nx*Ny). */
+int vk_fft2d(vk_gpe* h, int dir, const double* in, double* out);
+
+/* Padé-core vortex product initial condition (no reference counterpart:
+ * the reference reads its initial state from restart files).
+ * nv vortices at (xs, ys) with charges qs (+-1), healing length xi,
+ * rho(r) = r^2 (a1 + a2 r^2)/(1 + b1 r^2 + a2 r^4), r in units of xi, and
+ * the periodic image-sum phase; output psi_xy (Nx,Ny).  The _host variant
+ * runs on the CPU; vk_gpe_set_ic_vortices builds the field on the device
+ * and sets it as in vk_gpe_set_psi(h, ., flags). */
+int vk_ic_vortices_host(int N, int nv, const double* xs, const double* ys,
+                        const int* qs, double xi, int nimg, double* psi_xy);
+int vk_gpe_set_ic_vortices(vk_gpe* h, int nv, const double* xs,
+                           const double* ys, const int* qs, int nimg, int flags);
+
+/* Timing support.  stream: the handle's hipStream_t.  pass_times: runs
+ * nsteps RK4 steps with an event around every kernel launch and returns the
+ * average duration (ms) per kernel kind in ms_out[nkinds] (names via
+ * vk_pass_name); the state advances by nsteps. */
+void* vk_gpe_stream(vk_gpe* h);
+int vk_gpe_pass_times(vk_gpe* h, int nsteps, double* ms_out, int nkinds);
+const char* vk_pass_name(int kind);
+int vk_pass_kinds(void);
+/* Compulsory HBM bytes one launch of kernel kind moves at this handle's N. */
+double vk_gpe_pass_bytes(vk_gpe* h, int kind);
+
+const char* vk_last_error(void);
+const char* vk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+"""ctypes binding of the C ABI declared in include/vikshobha.h.
+
+The library is the product path: there is no CPU fallback.  Loading fails
+loudly when the built library is missing, and every compute entry point
+returns an error without a ROCm device.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libvikshobha.so")
+
+_lib = None
+
+dp = ctypes.POINTER(ctypes.c_double)
+ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Params(ctypes.Structure):
+    """vk_gpe_params"""
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dt", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("g", ctypes.c_double),
+        ("omega", ctypes.c_double),
+        ("disp", ctypes.c_double),
+        ("xi", ctypes.c_double),
+        ("np", ctypes.c_int),
+        ("V0", ctypes.c_double),
+        ("d", ctypes.c_double),
+        ("mass", ctypes.c_double),
+        ("force_dealias", ctypes.c_int),
+    ]
+
+
+# name -> (restype, argtypes); mirrors include/vikshobha.h
+SIGNATURES = {
+    "vk_gpe_params_init": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_int] + [ctypes.c_double] * 5),
+    "vk_nshell": (ctypes.c_int, [ctypes.c_int]),
+    "vk_gpe_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Params)]),
+    "vk_gpe_destroy": (None, [ctypes.c_void_p]),
+    "vk_gpe_set_psi": (ctypes.c_int, [ctypes.c_void_p, dp, ctypes.c_int]),
+    "vk_gpe_set_kpsi": (ctypes.c_int, [ctypes.c_void_p, dp]),
+    "vk_gpe_get_psi": (ctypes.c_int, [ctypes.c_void_p, dp, dp]),
+    "vk_gpe_rk4_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_gpe_rk4_steps_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_gpe_argle_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
+    "vk_gpe_diagnostics": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
+    "vk_gpe_particles_set": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
+    "vk_gpe_particles_get": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
+    "vk_fft2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
+    "vk_ic_vortices_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_double, ctypes.c_int, dp]),
+    "vk_gpe_set_ic_vortices": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, ip, ctypes.c_int, ctypes.c_int]),
+    "vk_gpe_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "vk_gpe_pass_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_int]),
+    "vk_pass_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "vk_pass_kinds": (ctypes.c_int, []),
+    "vk_gpe_pass_bytes": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_last_error": (ctypes.c_char_p, []),
+    "vk_version": (ctypes.c_char_p, []),
+}
+
+
+class VKError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "vikshobha native library not built (%s); run "
+                "`python -m quantumvortices_amd.build` or __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise VKError("vikshobha error %d: %s" % (rc, lib().vk_last_error().decode()))
+    return rc
+
+
+def ptr(a):
+    return a.ctypes.data_as(dp)

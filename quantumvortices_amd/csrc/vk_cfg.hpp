@@ -1,0 +1,79 @@
+// Per-size launch configuration and field layouts.
+//
+// Layouts in HBM (complex128 = 16 B):
+//   x-space  X[y*N + x]                     (the reference's psi(i1=x,i2=y))
+//   k-space  K[kx*N + ky]                   (the reference's kpsi(i2=ky,i1=kx),
+//                                            FFTW_MPI_TRANSPOSED_OUT)
+//   half-spectral H(y, kx) = DFT along x of a field, stored in panels of TC
+//   adjacent kx columns:  H[(p*N + y)*TC + c],  kx = p*TC + c.
+// A row pass (TR rows y per workgroup) reads/writes TR*TC*16 contiguous
+// bytes per panel; a column pass (one panel = TC columns per workgroup)
+// reads/writes one contiguous N*TC*16-byte block.
+#pragma once
+#include "vk_fft.hpp"
+
+namespace vk {
+
+template <int N_, int S_, int TC_, int TR_, bool SPLIT_>
+struct CfgT {
+    static constexpr int N = N_;
+    static constexpr int S = S_;          // points per thread per transform
+    static constexpr int T = N_ / S_;     // threads per transform
+    static constexpr int TC = TC_;        // columns per column-pass workgroup
+    static constexpr int TR = TR_;        // rows per row-pass workgroup
+    static constexpr int NP = N_ / TC_;   // panels
+    static constexpr bool SPLIT = SPLIT_;
+    static constexpr int LDSW = Lds<N_>::words * (SPLIT_ ? 1 : 2);  // doubles per transform
+    static constexpr int COLT = TC_ * T;  // column-pass threads
+    static constexpr int ROWT = TR_ * T;  // row-pass threads
+    static_assert(T % TC == 0, "TC must divide T");
+    static_assert(T >= 1, "N/S >= 1");
+};
+
+template <int N> struct Cfg;
+template <> struct Cfg<32> : CfgT<32, 8, 4, 4, true> {};
+template <> struct Cfg<64> : CfgT<64, 8, 8, 8, true> {};
+template <> struct Cfg<128> : CfgT<128, 8, 16, 16, true> {};
+template <> struct Cfg<256> : CfgT<256, 16, 16, 16, true> {};
+template <> struct Cfg<512> : CfgT<512, 16, 8, 8, true> {};
+template <> struct Cfg<1024> : CfgT<1024, 16, 4, 4, true> {};
+template <> struct Cfg<2048> : CfgT<2048, 16, 4, 4, true> {};
+template <> struct Cfg<4096> : CfgT<4096, 16, 2, 2, true> {};
+
+// Runtime panel geometry.  Retained |kx| <= M = floor(N/3) (the 2/3 rule
+// for 3 not dividing N).  Positive panels p < Pp cover kx in [0, M];
+// negative panels p >= NP - Pn cover kx in [N-M, N-1].
+struct Geo {
+    int M;
+    int Pp, Pn;
+};
+
+// Signed wavenumber of index i: (i-1) - N*(i/(N/2+1)) with 1-based i
+// (src/pspectral.f90:22); Nyquist maps to -N/2.
+__device__ __forceinline__ int ksg(int i, int N) { return i < N / 2 ? i : i - N; }
+
+// Dealias rule ksqr >= kalias**2 with kalias = N/3.0 (src/gpe.f90:346,
+// src/mod_grid.f90:92).  For 3 not dividing N, N^2/9 is never within
+// 1/9 of an integer, so the REAL comparison equals 9*ksqr >= N^2.
+__device__ __forceinline__ bool kmasked(int kx, int ky, int N) { return 9 * (kx * kx + ky * ky) >= N * N; }
+
+// panel pmode: 0 all, 1 retained both signs, 2 retained kx >= 0 only
+__device__ __forceinline__ bool panel_stored(int p, int NP, const Geo& g, int pmode)
+{
+    if (pmode == 0) return true;
+    if (pmode == 2) return p < g.Pp;
+    return p < g.Pp || p >= NP - g.Pn;
+}
+__device__ __forceinline__ int panel_of_block(int b, int NP, const Geo& g, int pmode)
+{
+    if (pmode == 1 && b >= g.Pp) return NP - g.Pn + (b - g.Pp);
+    return b;
+}
+
+template <class C>
+__device__ __forceinline__ size_t hoff(int p, int y, int c)
+{
+    return ((size_t)p * C::N + y) * C::TC + c;
+}
+
+}  // namespace vk

@@ -1,0 +1,862 @@
+// Host side of the vikshobha library: handle, pipelines, C ABI.
+//
+// One RK4 stage (src/gpe.f90:357-492 around gpe_rhs :498-671) is four
+// passes over HBM:
+//   P1 row   : psi(y,.) = IFFT_x Hpsi ; rho = |psi|^2 ; Hrho = FFT_x rho
+//   P2 column: rho-hat = FFT_y Hrho ; P_G ; Hrho_d = IFFT_y
+//   P3 row   : rho_d, V, psi rows ; N = (g rho_d + V) psi ; HN = FFT_x N
+//   P4 column: N-hat = FFT_y HN ; RHS, RK4 accumulation ; Hpsi = IFFT_y kpsi
+// kpsi is dealiased, so only |kx| <= N/3 columns are stored and moved;
+// rho, rho_d and V are real, so only their kx >= 0 halves are moved.
+#include "vk_kernels.hpp"
+#include "vikshobha.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <set>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+using vk::cplx;
+using vk::Geo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(VK_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum PassKind { PK_P1 = 0, PK_P2, PK_P3, PK_P4, PK_PART, PK_ARG4, PK_OTHER, PK_COUNT };
+const char* kPassNames[PK_COUNT] = {"p1_rho_row", "p2_dealias_col", "p3_nonlin_row", "p4_rk4_col",
+                                    "particles", "argle_update_col", "other"};
+
+int shell_of(int kx, int ky)
+{
+    // nint(sqrt(kx^2+ky^2)) (src/diagnostics.f90:341-343) by the integer
+    // identity  m(m-1) < s <= m(m+1)
+    const long long s = (long long)kx * kx + (long long)ky * ky;
+    if (s == 0) return 0;
+    long long m = (long long)std::sqrt((double)s);
+    while (m * (m + 1) < s) ++m;
+    while (m > 0 && (m - 1) * m >= s) --m;
+    return (int)m;
+}
+
+bool masked_host(int kx, int ky, int N) { return 9LL * ((long long)kx * kx + (long long)ky * ky) >= (long long)N * N; }
+int ksg_host(int i, int N) { return i < N / 2 ? i : i - N; }
+
+}  // namespace
+
+struct vk_gpe {
+    vk_gpe_params p{};
+    int N = 0;
+    int nshell = 0;
+    Geo geo{};
+    int NP = 0, TC = 0;
+    hipStream_t stream = nullptr;
+    cplx* tw = nullptr;
+    // state and RK4/ARGLE work fields
+    cplx *K0 = nullptr, *K1 = nullptr, *K2 = nullptr, *Hpsi = nullptr, *Hrho = nullptr, *HN = nullptr;
+    cplx *HV = nullptr, *KV = nullptr, *Krho = nullptr, *Vhat = nullptr, *EX = nullptr, *EY = nullptr;
+    double *fpart = nullptr, *pst = nullptr, *part = nullptr, *dsum = nullptr;
+    int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
+    int kv_valid = 0;
+    int have_part = 0;
+    // diagnostics
+    cplx* X[7] = {};
+    cplx* Kd[5] = {};
+    cplx* Hd = nullptr;
+    int *modes = nullptr, *offs = nullptr;
+    double *dspec = nullptr, *dpart = nullptr;
+    int diag_ready = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> evkind;
+    size_t evn = 0;
+};
+
+namespace {
+
+size_t f2(const vk_gpe* h) { return (size_t)h->N * h->N; }
+
+template <int N> using CfgN = vk::Cfg<N>;
+
+template <class F>
+int dispatch(int N, F&& f)
+{
+    switch (N) {
+    case 32: return f(std::integral_constant<int, 32>{});
+    case 64: return f(std::integral_constant<int, 64>{});
+    case 128: return f(std::integral_constant<int, 128>{});
+    case 256: return f(std::integral_constant<int, 256>{});
+    case 512: return f(std::integral_constant<int, 512>{});
+    case 1024: return f(std::integral_constant<int, 1024>{});
+    case 2048: return f(std::integral_constant<int, 2048>{});
+    case 4096: return f(std::integral_constant<int, 4096>{});
+    default: return fail(VK_ERR_UNSUPPORTED, "N must be a power of two in [32, 4096]");
+    }
+}
+
+void ev_begin(vk_gpe* h)
+{
+    if (!h->timing) return;
+    if (h->evn + 2 > h->ev.size()) {
+        for (int i = 0; i < 64; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) == hipSuccess) h->ev.push_back(e);
+        }
+    }
+    hipEventRecord(h->ev[h->evn], h->stream);
+}
+void ev_end(vk_gpe* h, int kind)
+{
+    if (!h->timing) return;
+    hipEventRecord(h->ev[h->evn + 1], h->stream);
+    h->evkind.push_back(kind);
+    h->evn += 2;
+}
+
+std::mutex g_lds_mu;
+std::set<const void*> g_lds_done;
+
+template <class Kern>
+void set_lds(Kern k, size_t bytes)
+{
+    if (bytes <= 65536) return;
+    std::lock_guard<std::mutex> lk(g_lds_mu);
+    if (g_lds_done.insert((const void*)k).second)
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <class C, class Kern, class... A>
+void row_launch(vk_gpe* h, int kind, Kern k, A... args)
+{
+    const size_t lds = (size_t)C::TR * C::LDSW * sizeof(double);
+    set_lds(k, lds);
+    ev_begin(h);
+    hipLaunchKernelGGL(k, dim3(C::N / C::TR), dim3(C::ROWT), lds, h->stream, args...);
+    ev_end(h, kind);
+}
+
+template <class C, class Kern, class... A>
+void col_launch(vk_gpe* h, int kind, int nblocks, Kern k, A... args)
+{
+    const size_t lds = (size_t)C::TC * C::LDSW * sizeof(double);
+    set_lds(k, lds);
+    ev_begin(h);
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(C::COLT), lds, h->stream, args...);
+    ev_end(h, kind);
+}
+
+int ncols(const vk_gpe* h, int pmode)
+{
+    if (pmode == 0) return h->NP;
+    if (pmode == 2) return h->geo.Pp;
+    return h->geo.Pp + h->geo.Pn;
+}
+
+double scale_of(const vk_gpe* h) { return 1.0 / ((double)h->N * (double)h->N); }
+
+// ---- generic 2D transforms ------------------------------------------------
+
+int fwd2d(vk_gpe* h, const cplx* X, cplx* Hs, cplx* K, int op_row, int op_col)
+{
+    return dispatch(h->N, [&](auto n) {
+        using C = CfgN<decltype(n)::value>;
+        row_launch<C>(h, PK_OTHER, vk::k_row_fwd<C>, X, Hs, (const cplx*)h->tw, h->geo, 0, op_row);
+        col_launch<C>(h, PK_OTHER, h->NP, vk::k_col_fwd<C>, (const cplx*)Hs, K, (const cplx*)h->tw, h->geo, 0, op_col);
+        return 0;
+    });
+}
+
+int col_inv(vk_gpe* h, const cplx* K, cplx* Hs, int pmode, int op)
+{
+    return dispatch(h->N, [&](auto n) {
+        using C = CfgN<decltype(n)::value>;
+        col_launch<C>(h, PK_OTHER, ncols(h, pmode), vk::k_col_inv<C>, K, Hs, (const cplx*)h->tw, h->geo, pmode, op);
+        return 0;
+    });
+}
+
+int row_inv(vk_gpe* h, const cplx* Hs, cplx* X, int pload, int op, double scale)
+{
+    return dispatch(h->N, [&](auto n) {
+        using C = CfgN<decltype(n)::value>;
+        row_launch<C>(h, PK_OTHER, vk::k_row_inv<C>, Hs, X, (const cplx*)h->tw, h->geo, pload, op, scale);
+        return 0;
+    });
+}
+
+int dealias_k(vk_gpe* h, cplx* K)
+{
+    hipLaunchKernelGGL(vk::k_dealias, dim3(1024), dim3(256), 0, h->stream, K, h->N);
+    return 0;
+}
+
+// ---- state management -------------------------------------------------------
+
+int ensure_any(vk_gpe* h)
+{
+    if (h->hstate == 0) {
+        int rc = col_inv(h, h->K0, h->Hpsi, 0, 0);
+        if (rc) return rc;
+        h->hstate = 1;
+    }
+    return 0;
+}
+
+int ensure_rk4(vk_gpe* h)
+{
+    if (h->hstate != 2) {
+        dealias_k(h, h->K0);
+        int rc = col_inv(h, h->K0, h->Hpsi, 1, 0);
+        if (rc) return rc;
+        h->hstate = 2;
+    }
+    return 0;
+}
+
+int ensure_full(vk_gpe* h)
+{
+    if (h->hstate != 1) {
+        int rc = col_inv(h, h->K0, h->Hpsi, 0, 0);
+        if (rc) return rc;
+        h->hstate = 1;
+    }
+    return 0;
+}
+
+// ---- particles ----------------------------------------------------------------
+
+double ipow(double x, int n)
+{
+    double r = 1.0;
+    for (int i = 0; i < n; ++i) r *= x;
+    return r;
+}
+
+vk::PartArgs part_args(const vk_gpe* h, int nrows, int half, int stage)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    const double L = 2.0 * pi;
+    const double dx = L / h->N;
+    vk::PartArgs a{};
+    a.np = h->p.np;
+    a.N = h->N;
+    a.nrows = nrows;
+    a.half = half;
+    a.coef = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
+    const double dE = (double)0.062f;         // src/gpe_particle.f90:395, default-real literal
+    const double r0 = 4.0 * h->p.xi;         // :396
+    a.rf = 384.0 * dE * ipow(pi, 12) * ipow(r0, 12) / ipow(L, 12);
+    a.efac = 64.0 * dE * ipow(pi, 12) * ipow(r0, 12) / ipow(L, 12);
+    a.dt = h->p.dt;
+    a.mass = h->p.mass;
+    a.stage = stage;
+    return a;
+}
+
+// KV on rows [0, nrows) from the current positions
+int particle_kv(vk_gpe* h, int nrows)
+{
+    const int np = h->p.np;
+    ev_begin(h);
+    hipLaunchKernelGGL(vk::k_ptab, dim3((np * h->N + 255) / 256), dim3(256), 0, h->stream, (const double*)h->pst,
+                       h->EX, h->EY, np, h->N);
+    hipLaunchKernelGGL(vk::k_kv, dim3(nrows), dim3(256), (size_t)np * sizeof(cplx), h->stream,
+                       (const cplx*)h->Vhat, (const cplx*)h->EX, (const cplx*)h->EY, h->KV, np, h->N, nrows);
+    ev_end(h, PK_PART);
+    return 0;
+}
+
+int particle_forces(vk_gpe* h, const cplx* Krho, int nrows, int half, int stage)
+{
+    const int np = h->p.np;
+    constexpr int PG = 8;
+    ev_begin(h);
+    hipLaunchKernelGGL(vk::k_force_rows<PG>, dim3(nrows, (np + PG - 1) / PG), dim3(256), 0, h->stream, Krho,
+                       (const cplx*)h->Vhat, (const cplx*)h->EY, h->fpart, np, h->N);
+    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, h->stream, (const double*)h->fpart, (const cplx*)h->EX,
+                       h->pst, part_args(h, nrows, half, stage));
+    ev_end(h, PK_PART);
+    return 0;
+}
+
+// ---- RK4 / ARGLE -----------------------------------------------------------------
+
+int rk4_step(vk_gpe* h)
+{
+    return dispatch(h->N, [&](auto n) {
+        using C = CfgN<decltype(n)::value>;
+        const double sc = scale_of(h);
+        const double dt = h->p.dt, oney6 = 1.0 / 6.0;
+        const int parts = h->have_part && h->p.np > 0;
+        const int Kp = C::TC * h->geo.Pp;
+        vk::RKArgs a{};
+        a.pref = make_double2(-h->p.disp, -1.0);   // (1 - i disp)/i
+        a.alpha = h->p.alpha;
+        a.omega = h->p.omega;
+        for (int stage = 1; stage <= 4; ++stage) {
+            if (parts) particle_kv(h, Kp);
+            row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc);
+            col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
+                          (const cplx*)h->tw, h->geo, parts ? 3 : 0);
+            if (parts) particle_forces(h, h->Krho, Kp, 1, stage);
+            row_launch<C>(h, PK_P3, vk::k_p3<C>, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
+                          h->HN, (const cplx*)h->tw, h->geo, 1, 1, h->p.g * sc, sc, sc, parts ? 1 : 0);
+            a.stage = stage;
+            a.wacc = (stage == 1 || stage == 4) ? dt * oney6 : dt * oney6 * 2.0;
+            a.cstep = stage == 3 ? dt : dt * 0.5;
+            col_launch<C>(h, PK_P4, h->geo.Pp + h->geo.Pn, vk::k_p4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
+                          h->K2, (const cplx*)h->tw, h->geo, a);
+        }
+        h->kv_valid = 0;
+        return 0;
+    });
+}
+
+int argle_step(vk_gpe* h, int want_res)
+{
+    return dispatch(h->N, [&](auto n) {
+        using C = CfgN<decltype(n)::value>;
+        const double sc = scale_of(h);
+        const int parts = h->have_part && h->p.np > 0;
+        if (parts && !h->kv_valid) {
+            particle_kv(h, C::TC * h->geo.Pp);
+            h->kv_valid = 1;
+        }
+        row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc);
+        col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
+                      (const cplx*)h->tw, h->geo, parts ? 1 : 0);
+        row_launch<C>(h, PK_P3, vk::k_p3<C>, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi, h->HN,
+                      (const cplx*)h->tw, h->geo, 0, 0, -h->p.g * sc, -sc, sc, parts ? 1 : 0);
+        vk::ArgleArgs a{h->p.alpha, h->p.omega, h->p.dt, want_res};
+        col_launch<C>(h, PK_ARG4, h->NP, vk::k_a4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, (const cplx*)h->tw, a,
+                      h->part);
+        if (want_res)
+            hipLaunchKernelGGL(vk::k_reduce_fixed, dim3(1), dim3(256), 0, h->stream, (const double*)h->part, h->NP, 2,
+                               h->dsum);
+        return 0;
+    });
+}
+
+// ---- diagnostics ---------------------------------------------------------------
+
+int diag_setup(vk_gpe* h)
+{
+    if (h->diag_ready) return 0;
+    const size_t fb = f2(h) * sizeof(cplx);
+    for (auto& x : h->X) HIPCHK(hipMalloc(&x, fb));
+    for (auto& k : h->Kd) HIPCHK(hipMalloc(&k, fb));
+    HIPCHK(hipMalloc(&h->Hd, fb));
+    HIPCHK(hipMemset(h->Hd, 0, fb));
+    // retained modes grouped by shell, in the reference's loop order
+    // (kx index outer, ky index inner; src/diagnostics.f90:332-402)
+    const int N = h->N, ns = h->nshell;
+    std::vector<int> cnt(ns + 2, 0);
+    for (int i1 = 0; i1 < N; ++i1)
+        for (int i2 = 0; i2 < N; ++i2) {
+            const int kx = ksg_host(i1, N), ky = ksg_host(i2, N);
+            if (masked_host(kx, ky, N)) continue;
+            cnt[shell_of(kx, ky) + 1]++;
+        }
+    for (int m = 0; m <= ns; ++m) cnt[m + 1] += cnt[m];
+    std::vector<int> modes(cnt[ns + 1]);
+    std::vector<int> pos(cnt.begin(), cnt.end());
+    for (int i1 = 0; i1 < N; ++i1)
+        for (int i2 = 0; i2 < N; ++i2) {
+            const int kx = ksg_host(i1, N), ky = ksg_host(i2, N);
+            if (masked_host(kx, ky, N)) continue;
+            modes[pos[shell_of(kx, ky)]++] = i1 * N + i2;
+        }
+    HIPCHK(hipMalloc(&h->modes, sizeof(int) * std::max<size_t>(1, modes.size())));
+    HIPCHK(hipMalloc(&h->offs, sizeof(int) * (ns + 2)));
+    HIPCHK(hipMemcpy(h->modes, modes.data(), sizeof(int) * modes.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->offs, cnt.data(), sizeof(int) * (ns + 2), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&h->dspec, sizeof(double) * 6 * (ns + 1)));
+    HIPCHK(hipMalloc(&h->dpart, sizeof(double) * 8 * 1024));
+    h->diag_ready = 1;
+    return 0;
+}
+
+int diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal)
+{
+    int rc = diag_setup(h);
+    if (rc) return rc;
+    if ((rc = ensure_any(h))) return rc;
+    const double sc = scale_of(h);
+    const int pl = h->hstate == 2 ? 1 : 0;
+    cplx **X = h->X, **Kd = h->Kd;
+    // psi, d_x psi, d_y psi
+    if ((rc = row_inv(h, h->Hpsi, X[0], pl, 0, sc))) return rc;
+    if ((rc = row_inv(h, h->Hpsi, X[1], pl, 1, sc))) return rc;
+    if ((rc = col_inv(h, h->K0, h->Hd, pl, 2))) return rc;
+    if ((rc = row_inv(h, h->Hd, X[2], pl, 0, sc))) return rc;
+    // rho-hat (dealiased) -> Kd[4]; P_G rho -> X[3]
+    if ((rc = fwd2d(h, X[0], h->Hd, Kd[4], 1, 1))) return rc;
+    if ((rc = col_inv(h, Kd[4], h->Hd, 0, 0))) return rc;
+    if ((rc = row_inv(h, h->Hd, X[3], 0, 0, sc))) return rc;
+    const int parts = h->have_part && h->p.np > 0;
+    if (parts) {
+        particle_kv(h, h->N);   // all rows
+        if ((rc = col_inv(h, h->KV, h->Hd, 0, 0))) return rc;
+        if ((rc = row_inv(h, h->Hd, X[4], 0, 0, sc))) return rc;
+    }
+    const double pi = 4.0 * std::atan(1.0);
+    vk::DiagArgs da{h->p.alpha, h->p.omega, h->p.g, (double)1e-30f, h->N, parts};
+    const int nb = 1024;
+    hipLaunchKernelGGL(vk::k_diag_x, dim3(nb), dim3(256), 0, h->stream, (const cplx*)X[0], (const cplx*)X[1],
+                       (const cplx*)X[2], (const cplx*)X[3], (const cplx*)X[4], X[1], X[2], X[5], X[6], da, h->dpart);
+    hipLaunchKernelGGL(vk::k_reduce_fixed, dim3(1), dim3(256), 0, h->stream, (const double*)h->dpart, nb, 8, h->dsum);
+    // spectra: FFT of w_x, w_y, q_x, q_y
+    if ((rc = fwd2d(h, X[1], h->Hd, Kd[0], 0, 0))) return rc;
+    if ((rc = fwd2d(h, X[2], h->Hd, Kd[1], 0, 0))) return rc;
+    if ((rc = fwd2d(h, X[5], h->Hd, Kd[2], 0, 0))) return rc;
+    if ((rc = fwd2d(h, X[6], h->Hd, Kd[3], 0, 0))) return rc;
+    const double en = 1.0 / ((double)h->N * (double)h->N);
+    vk::SpecArgs sa{h->p.alpha, h->p.g, en * en, h->N, h->nshell};
+    hipLaunchKernelGGL(vk::k_spectra, dim3(h->nshell + 1), dim3(256), 0, h->stream, (const int*)h->modes,
+                       (const int*)h->offs, (const cplx*)Kd[0], (const cplx*)Kd[1], (const cplx*)Kd[2],
+                       (const cplx*)Kd[3], (const cplx*)Kd[4], (const cplx*)h->K0, sa, h->dspec);
+    if (parts) particle_forces(h, Kd[4], h->N, 0, 0);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    double s[8];
+    HIPCHK(hipMemcpy(s, h->dsum, sizeof(s), hipMemcpyDeviceToHost));
+    std::vector<double> sp(6 * (h->nshell + 1));
+    HIPCHK(hipMemcpy(sp.data(), h->dspec, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
+    const double L = 2.0 * pi, dx = L / h->N, dvol = dx * dx, L2 = L * L, al = h->p.alpha;
+    // final factors (src/diagnostics.f90:105-106,189-207,244)
+    const double egrad = 2.0 * al * s[2] * dvol / L2;
+    const double echem = 2.0 * al * s[3] * dvol / L2;
+    const double eintr = 2.0 * al * s[4] * dvol / L2;
+    if (scal) {
+        scal[0] = egrad;
+        scal[1] = echem;
+        scal[2] = eintr;
+        scal[3] = egrad + eintr;
+        scal[4] = s[6] * dvol;
+        scal[5] = 2.0 * al * s[0] * dvol / L2;
+        scal[6] = 2.0 * al * s[1] * dvol / L2;
+        scal[7] = sp[5 * (h->nshell + 1) + 0];
+    }
+    if (spec) std::memcpy(spec, sp.data(), sizeof(double) * sp.size());
+    if (pscal) {
+        for (int i = 0; i < 5; ++i) pscal[i] = 0.0;
+        if (parts) {
+            double dg[4];
+            HIPCHK(hipMemcpy(dg, h->pst + 28 * h->p.np, sizeof(dg), hipMemcpyDeviceToHost));
+            pscal[0] = 2.0 * al * s[5] * dvol / L2;
+            pscal[1] = dg[0];
+            pscal[2] = dg[1];
+            pscal[3] = dg[2];
+            pscal[4] = dg[3];
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+
+extern "C" {
+
+const char* vk_last_error(void) { return g_err.c_str(); }
+const char* vk_version(void) { return "vikshobha-mi355x 0.1 (gfx950)"; }
+
+int vk_nshell(int N)
+{
+    const float a = 1.414f * (float)N;   // default-real literal (src/mod_grid.f90:77)
+    return (int)((double)a / 2.0) + 1;
+}
+
+int vk_gpe_params_init(vk_gpe_params* p, int N, double dt, double xi_in, double csound, double rho_avg, double disp)
+{
+    if (!p || N < 32) return fail(VK_ERR_ARG, "params_init: bad arguments");
+    std::memset(p, 0, sizeof(*p));
+    const double pi = 4.0 * std::atan(1.0);
+    const double L = 2.0 * pi;
+    const double dx = L / N;
+    p->N = N;
+    p->dt = dt;
+    p->xi = xi_in * dx;
+    p->alpha = csound * p->xi * std::sqrt(rho_avg / 2.0);
+    p->g = (csound * csound) / (2.0 * p->alpha);
+    p->omega = p->g;
+    p->disp = disp;
+    p->mass = 1.0;
+    p->force_dealias = 1;
+    return 0;
+}
+
+int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
+{
+    if (!out || !p) return fail(VK_ERR_ARG, "create: null argument");
+    *out = nullptr;
+    const int N = p->N;
+    if (N < 32 || N > 4096 || (N & (N - 1))) return fail(VK_ERR_UNSUPPORTED, "N must be a power of two in [32, 4096]");
+    if (p->np < 0 || p->np > 256) return fail(VK_ERR_ARG, "np must be in [0, 256]");
+    if (p->np > 0 && !p->force_dealias)
+        return fail(VK_ERR_UNSUPPORTED, "force_dealias=0 (no dealiasing of rho-hat in the particle force) is not implemented");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(VK_ERR_HIP, "no ROCm device available");
+    vk_gpe* h = new vk_gpe();
+    h->p = *p;
+    h->N = N;
+    h->nshell = vk_nshell(N);
+    int TC = 0;
+    dispatch(N, [&](auto n) { TC = CfgN<decltype(n)::value>::TC; return 0; });
+    h->TC = TC;
+    h->NP = N / TC;
+    const int M = N / 3;
+    h->geo.M = M;
+    h->geo.Pp = (M + 1 + TC - 1) / TC;
+    h->geo.Pn = (M + TC - 1) / TC;
+    if (h->geo.Pp + h->geo.Pn > h->NP) { h->geo.Pp = h->NP; h->geo.Pn = 0; }
+    auto cleanup = [&](int rc) { vk_gpe_destroy(h); return rc; };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(VK_ERR_HIP, "hipStreamCreate failed"));
+    const size_t fb = f2(h) * sizeof(cplx);
+    cplx** bufs[] = {&h->K0, &h->K1, &h->K2, &h->Hpsi, &h->Hrho, &h->HN};
+    for (auto b : bufs) {
+        if (hipMalloc(b, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (fields)"));
+        hipMemset(*b, 0, fb);
+    }
+    if (hipMalloc(&h->tw, sizeof(cplx) * N) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc tw"));
+    // twiddles w_k = exp(-2 pi i k/N): long-double sin/cos on the first
+    // octant, exact quadrant/octant symmetries elsewhere
+    {
+        std::vector<double2> tw(N);
+        const long double pil = 3.141592653589793238462643383279502884L;
+        for (int k = 0; k < N; ++k) {
+            const int q = N / 4, quad = k / q;
+            int a = k % q, flip = 0;
+            if (2 * a > q) { a = q - a; flip = 1; }
+            const long double t = 2.0L * pil * (long double)a / (long double)N;
+            long double c = cosl(t), s = sinl(t);
+            if (flip) { long double tmp = c; c = s; s = tmp; }
+            for (int i = 0; i < quad; ++i) { long double tmp = c; c = -s; s = tmp; }
+            tw[k] = make_double2((double)c, -(double)s);
+        }
+        if (hipMemcpy(h->tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "twiddle upload"));
+    }
+    if (hipMalloc(&h->part, sizeof(double) * 2 * h->NP) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc part"));
+    if (hipMalloc(&h->dsum, sizeof(double) * 8) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc dsum"));
+    if (p->np > 0) {
+        const int np = p->np;
+        cplx** pb[] = {&h->HV, &h->KV, &h->Krho, &h->Vhat};
+        for (auto b : pb) {
+            if (hipMalloc(b, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particles)"));
+            hipMemset(*b, 0, fb);
+        }
+        if (hipMalloc(&h->EX, sizeof(cplx) * np * N) != hipSuccess ||
+            hipMalloc(&h->EY, sizeof(cplx) * np * N) != hipSuccess ||
+            hipMalloc(&h->fpart, sizeof(double) * 4 * np * N) != hipSuccess ||
+            hipMalloc(&h->pst, sizeof(double) * (28 * np + 8)) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particle tables)"));
+        hipMemset(h->pst, 0, sizeof(double) * (28 * np + 8));
+    } else {
+        // P2/P3 take these pointers even when unused
+        h->HV = h->Hrho;
+        h->KV = h->K1;
+        h->Krho = h->K1;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) return cleanup(fail(VK_ERR_HIP, "create: device sync"));
+    *out = h;
+    return 0;
+}
+
+void vk_gpe_destroy(vk_gpe* h)
+{
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd};
+    for (auto f : fields) if (f) hipFree(f);
+    if (h->p.np > 0) {
+        cplx* pf[] = {h->HV, h->KV, h->Krho, h->Vhat, h->EX, h->EY};
+        for (auto f : pf) if (f) hipFree(f);
+        if (h->fpart) hipFree(h->fpart);
+        if (h->pst) hipFree(h->pst);
+    }
+    for (auto x : h->X) if (x) hipFree(x);
+    for (auto k : h->Kd) if (k) hipFree(k);
+    if (h->modes) hipFree(h->modes);
+    if (h->offs) hipFree(h->offs);
+    if (h->dspec) hipFree(h->dspec);
+    if (h->dpart) hipFree(h->dpart);
+    if (h->part) hipFree(h->part);
+    if (h->dsum) hipFree(h->dsum);
+    for (auto e : h->ev) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static int need_x0(vk_gpe* h)
+{
+    if (!h->X[0]) HIPCHK(hipMalloc(&h->X[0], f2(h) * sizeof(cplx)));
+    if (!h->Hd) {
+        HIPCHK(hipMalloc(&h->Hd, f2(h) * sizeof(cplx)));
+        HIPCHK(hipMemset(h->Hd, 0, f2(h) * sizeof(cplx)));
+    }
+    return 0;
+}
+
+static int set_from_x0(vk_gpe* h, int flags)
+{
+    int rc = fwd2d(h, h->X[0], h->Hd, h->K0, 0, 0);
+    if (rc) return rc;
+    h->hstate = 0;
+    if (flags & VK_SYNC_DEALIAS) {
+        if ((rc = ensure_rk4(h))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vk_gpe_set_psi(vk_gpe* h, const double* psi, int flags)
+{
+    if (!h || !psi) return fail(VK_ERR_ARG, "set_psi: null argument");
+    int rc = need_x0(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(h->X[0], psi, f2(h) * sizeof(cplx), hipMemcpyHostToDevice));
+    return set_from_x0(h, flags);
+}
+
+int vk_gpe_set_kpsi(vk_gpe* h, const double* kpsi)
+{
+    if (!h || !kpsi) return fail(VK_ERR_ARG, "set_kpsi: null argument");
+    HIPCHK(hipMemcpy(h->K0, kpsi, f2(h) * sizeof(cplx), hipMemcpyHostToDevice));
+    h->hstate = 0;
+    return 0;
+}
+
+int vk_gpe_get_psi(vk_gpe* h, double* psi, double* kpsi)
+{
+    if (!h) return fail(VK_ERR_ARG, "get_psi: null handle");
+    int rc;
+    if (psi) {
+        if ((rc = need_x0(h))) return rc;
+        if ((rc = ensure_any(h))) return rc;
+        if ((rc = row_inv(h, h->Hpsi, h->X[0], h->hstate == 2 ? 1 : 0, 0, scale_of(h)))) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(psi, h->X[0], f2(h) * sizeof(cplx), hipMemcpyDeviceToHost));
+    }
+    if (kpsi) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(kpsi, h->K0, f2(h) * sizeof(cplx), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int vk_gpe_rk4_steps_async(vk_gpe* h, int nsteps)
+{
+    if (!h || nsteps < 0) return fail(VK_ERR_ARG, "rk4_steps: bad arguments");
+    int rc = ensure_rk4(h);
+    if (rc) return rc;
+    for (int i = 0; i < nsteps; ++i)
+        if ((rc = rk4_step(h))) return rc;
+    return 0;
+}
+
+int vk_gpe_rk4_steps(vk_gpe* h, int nsteps)
+{
+    int rc = vk_gpe_rk4_steps_async(h, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vk_gpe_argle_steps(vk_gpe* h, int nsteps, double* residual)
+{
+    if (!h || nsteps < 0) return fail(VK_ERR_ARG, "argle_steps: bad arguments");
+    int rc = ensure_full(h);
+    if (rc) return rc;
+    for (int i = 0; i < nsteps; ++i)
+        if ((rc = argle_step(h, residual && i == nsteps - 1))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    if (residual && nsteps > 0) {
+        double r[2];
+        HIPCHK(hipMemcpy(r, h->dsum, sizeof(r), hipMemcpyDeviceToHost));
+        *residual = std::sqrt(r[0]) / (h->p.dt * std::sqrt(r[1]));
+    }
+    return 0;
+}
+
+int vk_gpe_diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal)
+{
+    if (!h) return fail(VK_ERR_ARG, "diagnostics: null handle");
+    return diagnostics(h, scal, spec, pscal);
+}
+
+int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel, const double* fext)
+{
+    if (!h) return fail(VK_ERR_ARG, "particles_set: null handle");
+    const int np = h->p.np;
+    if (np <= 0) return fail(VK_ERR_STATE, "particles_set: handle created with np = 0");
+    int rc = need_x0(h);
+    if (rc) return rc;
+    if (pos) HIPCHK(hipMemcpy(h->pst, pos, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
+    if (vel) HIPCHK(hipMemcpy(h->pst + 2 * np, vel, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
+    if (fext) HIPCHK(hipMemcpy(h->pst + 4 * np, fext, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
+    if (!h->have_part) {
+        // particle_potential (src/gpe_particle.f90:3-90): V on the grid,
+        // forward transform, dealias -> Vhat
+        const double pi = 4.0 * std::atan(1.0);
+        const double L = 2.0 * pi, dx = L / h->N, c1 = 2 * pi / (2.0 * L);
+        hipLaunchKernelGGL(vk::k_pot_x, dim3(1024), dim3(256), 0, h->stream, h->X[0], h->N, dx, c1, h->p.V0, h->p.d);
+        if ((rc = fwd2d(h, h->X[0], h->Hd, h->Vhat, 0, 1))) return rc;
+        h->have_part = 1;
+    }
+    h->kv_valid = 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vk_gpe_particles_get(vk_gpe* h, double* pos, double* vel, double* frc)
+{
+    if (!h) return fail(VK_ERR_ARG, "particles_get: null handle");
+    const int np = h->p.np;
+    if (np <= 0) return fail(VK_ERR_STATE, "particles_get: handle created with np = 0");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (pos) HIPCHK(hipMemcpy(pos, h->pst, sizeof(double) * 2 * np, hipMemcpyDeviceToHost));
+    if (vel) HIPCHK(hipMemcpy(vel, h->pst + 2 * np, sizeof(double) * 2 * np, hipMemcpyDeviceToHost));
+    if (frc) HIPCHK(hipMemcpy(frc, h->pst + 26 * np, sizeof(double) * 2 * np, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vk_fft2d(vk_gpe* h, int dir, const double* in, double* out)
+{
+    if (!h || !in || !out || (dir != -1 && dir != 1)) return fail(VK_ERR_ARG, "fft2d: bad arguments");
+    int rc = need_x0(h);
+    if (rc) return rc;
+    if (!h->Kd[0]) HIPCHK(hipMalloc(&h->Kd[0], f2(h) * sizeof(cplx)));
+    const size_t fb = f2(h) * sizeof(cplx);
+    if (dir < 0) {
+        HIPCHK(hipMemcpy(h->X[0], in, fb, hipMemcpyHostToDevice));
+        if ((rc = fwd2d(h, h->X[0], h->Hd, h->Kd[0], 0, 0))) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(out, h->Kd[0], fb, hipMemcpyDeviceToHost));
+    } else {
+        HIPCHK(hipMemcpy(h->Kd[0], in, fb, hipMemcpyHostToDevice));
+        if ((rc = col_inv(h, h->Kd[0], h->Hd, 0, 0))) return rc;
+        if ((rc = row_inv(h, h->Hd, h->X[0], 0, 0, scale_of(h)))) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(out, h->X[0], fb, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int vk_ic_vortices_host(int N, int nv, const double* xs, const double* ys, const int* qs, double xi, int nimg,
+                        double* psi)
+{
+    if (N < 2 || nv < 0 || !psi || (nv > 0 && (!xs || !ys || !qs))) return fail(VK_ERR_ARG, "ic: bad arguments");
+    const double dx = 2.0 * 3.14159265358979323846 / N;
+    for (int iy = 0; iy < N; ++iy)
+        for (int ix = 0; ix < N; ++ix) vk::ic_point(ix * dx, iy * dx, nv, xs, ys, qs, xi, nimg, psi + 2 * ((size_t)iy * N + ix));
+    return 0;
+}
+
+int vk_gpe_set_ic_vortices(vk_gpe* h, int nv, const double* xs, const double* ys, const int* qs, int nimg, int flags)
+{
+    if (!h || nv < 0 || (nv > 0 && (!xs || !ys || !qs))) return fail(VK_ERR_ARG, "set_ic_vortices: bad arguments");
+    int rc = need_x0(h);
+    if (rc) return rc;
+    double *dx = nullptr, *dy = nullptr;
+    int* dq = nullptr;
+    const int n = nv > 0 ? nv : 1;
+    HIPCHK(hipMalloc(&dx, sizeof(double) * n));
+    HIPCHK(hipMalloc(&dy, sizeof(double) * n));
+    HIPCHK(hipMalloc(&dq, sizeof(int) * n));
+    if (nv > 0) {
+        HIPCHK(hipMemcpy(dx, xs, sizeof(double) * nv, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dy, ys, sizeof(double) * nv, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dq, qs, sizeof(int) * nv, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(vk::k_ic, dim3(2048), dim3(256), 0, h->stream, h->X[0], h->N, nv, (const double*)dx,
+                       (const double*)dy, (const int*)dq, h->p.xi, nimg);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    hipFree(dx);
+    hipFree(dy);
+    hipFree(dq);
+    return set_from_x0(h, flags);
+}
+
+void* vk_gpe_stream(vk_gpe* h) { return h ? (void*)h->stream : nullptr; }
+
+int vk_pass_kinds(void) { return PK_COUNT; }
+const char* vk_pass_name(int kind) { return (kind >= 0 && kind < PK_COUNT) ? kPassNames[kind] : "?"; }
+
+int vk_gpe_pass_times(vk_gpe* h, int nsteps, double* ms_out, int nkinds)
+{
+    if (!h || nsteps <= 0 || !ms_out) return fail(VK_ERR_ARG, "pass_times: bad arguments");
+    int rc = ensure_rk4(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->timing = true;
+    h->evn = 0;
+    h->evkind.clear();
+    for (int i = 0; i < nsteps && !rc; ++i) rc = rk4_step(h);
+    h->timing = false;
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    std::vector<double> sum(PK_COUNT, 0.0);
+    std::vector<int> cnt(PK_COUNT, 0);
+    for (size_t i = 0; i < h->evkind.size(); ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
+        sum[h->evkind[i]] += ms;
+        cnt[h->evkind[i]]++;
+    }
+    for (int k = 0; k < nkinds && k < PK_COUNT; ++k) ms_out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    return 0;
+}
+
+double vk_gpe_pass_bytes(vk_gpe* h, int kind)
+{
+    if (!h) return 0.0;
+    const double F = 16.0 * (double)h->N * (double)h->N;
+    const double fp = (double)(h->geo.Pp * h->TC) / h->N;                        // kx >= 0 half columns
+    const double fr = (double)((h->geo.Pp + h->geo.Pn) * h->TC) / h->N;         // retained columns
+    // retained disk modes
+    long long disk = 0;
+    for (int kx = -h->N / 2; kx < h->N / 2; ++kx)
+        for (int ky = -h->N / 2; ky < h->N / 2; ++ky)
+            if (!masked_host(kx, ky, h->N)) ++disk;
+    const double D = 16.0 * (double)disk;
+    switch (kind) {
+    case PK_P1: return F * (fr + fp);
+    case PK_P2: return F * 2.0 * fp;
+    case PK_P3: return F * (fp + fr + fr);
+    case PK_P4: return F * (fr + fr) + D * (3 + 5 + 5 + 4) / 4.0;
+    default: return 0.0;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,825 @@
+// Device kernels of the GPE step.  Every transform pass is a row pass
+// (TR rows of N points along x per workgroup) or a column pass (TC columns
+// of N points along y per workgroup); pointwise physics is fused into the
+// passes so that each field round trip to HBM carries as much work as
+// possible.  See vk_cfg.hpp for the layouts.
+#pragma once
+#include "vk_cfg.hpp"
+
+namespace vk {
+
+// ----------------------------------------------------------------------------
+// thread <-> data mapping helpers
+// ----------------------------------------------------------------------------
+
+#define VK_ROW_SETUP(C)                                                    \
+    constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC, TR = C::TR;    \
+    constexpr int NP = C::NP;                                              \
+    (void)NP;                                                              \
+    extern __shared__ double lds[];                                        \
+    const int u = threadIdx.x;                                             \
+    const int q_ = u % (TR * TC);                                          \
+    const int rr = q_ / TC, c = q_ % TC, pidx0 = u / (TR * TC);            \
+    const int t = TC * pidx0 + c;                                          \
+    const int y = blockIdx.x * TR + rr;                                    \
+    double* lg = lds + rr * C::LDSW;
+
+#define VK_COL_SETUP(C, PEXPR)                                             \
+    constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC;                \
+    constexpr int NP = C::NP;                                              \
+    (void)NP;                                                              \
+    extern __shared__ double lds[];                                        \
+    const int u = threadIdx.x;                                             \
+    const int c = u % TC, t = u / TC;                                      \
+    const int p = (PEXPR);                                                 \
+    const int kx = p * TC + c;                                             \
+    double* lg = lds + c * C::LDSW;
+
+// panel holding kx = t + T*s in a row pass
+#define VK_PANEL(s) (pidx0 + (T / TC) * (s))
+
+__device__ __forceinline__ cplx czero() { return mk(0.0, 0.0); }
+
+// ----------------------------------------------------------------------------
+// Generic transform passes (set/get, diagnostics, conformance hook)
+// ----------------------------------------------------------------------------
+
+// x-space rows -> FFT along x -> H.  op 1: transform |X|^2 instead of X.
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) k_row_fwd(const cplx* __restrict__ X, cplx* __restrict__ H,
+                                                     const cplx* __restrict__ tw, Geo g, int pstore, int op)
+{
+    VK_ROW_SETUP(C)
+    cplx v[S];
+    const cplx* xr = X + (size_t)y * N;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        cplx a = xr[t + T * s];
+        v[s] = op == 1 ? mk(a.x * a.x + a.y * a.y, 0.0) : a;
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        if (panel_stored(pp, NP, g, pstore)) H[hoff<C>(pp, y, c)] = v[s];
+    }
+}
+
+// H column -> FFT along y -> K.  op 1: dealias.
+template <class C>
+__global__ void __launch_bounds__(C::COLT) k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
+                                                     const cplx* __restrict__ tw, Geo g, int pmode, int op)
+{
+    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, pmode))
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = H[hoff<C>(p, t + T * s, c)];
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    const int kxs = ksg(kx, N);
+    cplx* kr = K + (size_t)kx * N;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int ky = t + T * s;
+        cplx a = v[s];
+        if (op == 1 && kmasked(kxs, ksg(ky, N), N)) a = czero();
+        kr[ky] = a;
+    }
+}
+
+// K row -> op -> unnormalised inverse FFT along y -> H.
+// op 0: as is; 1: dealias; 2: times i*ky.
+template <class C>
+__global__ void __launch_bounds__(C::COLT) k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
+                                                     const cplx* __restrict__ tw, Geo g, int pmode, int op)
+{
+    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, pmode))
+    cplx v[S];
+    const int kxs = ksg(kx, N);
+    const cplx* kr = K + (size_t)kx * N;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int ky = t + T * s;
+        const int kys = ksg(ky, N);
+        cplx a = kr[ky];
+        if (op == 1 && kmasked(kxs, kys, N)) a = czero();
+        if (op == 2) a = mk(-(double)kys * a.y, (double)kys * a.x);
+        v[s] = conjc(a);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) H[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
+}
+
+// H row -> op -> unnormalised inverse FFT along x -> times scale -> X.
+// pload 2: kx >= 0 panels stored, negative kx from Hermitian symmetry.
+// op 1: times i*kx before the transform.
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) k_row_inv(const cplx* __restrict__ H, cplx* __restrict__ X,
+                                                     const cplx* __restrict__ tw, Geo g, int pload, int op,
+                                                     double scale)
+{
+    VK_ROW_SETUP(C)
+    cplx v[S];
+    const int Kp = TC * g.Pp;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int kxi = t + T * s;
+        cplx a = czero();
+        if (pload == 2) {
+            if (kxi < Kp) a = H[hoff<C>(kxi / TC, y, kxi % TC)];
+            else if (kxi > N - Kp) {
+                const int km = N - kxi;
+                a = conjc(H[hoff<C>(km / TC, y, km % TC)]);
+            }
+        } else {
+            const int pp = VK_PANEL(s);
+            if (panel_stored(pp, NP, g, pload)) a = H[hoff<C>(pp, y, c)];
+        }
+        if (op == 1) {
+            const double kk = (double)ksg(kxi, N);
+            a = mk(-kk * a.y, kk * a.x);
+        }
+        v[s] = conjc(a);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    cplx* xr = X + (size_t)y * N;
+#pragma unroll
+    for (int s = 0; s < S; ++s) xr[t + T * s] = rmul(scale, conjc(v[s]));
+}
+
+// zero the dealiased modes of a k-space field (all N^2 entries)
+__global__ void k_dealias(cplx* __restrict__ K, int N)
+{
+    const size_t n2 = (size_t)N * N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i / N), ky = (int)(i % N);
+        if (kmasked(ksg(kx, N), ksg(ky, N), N)) K[i] = czero();
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Fused RK4 / ARGLE passes
+// ----------------------------------------------------------------------------
+
+// P1 (row): psi row from Hpsi (inverse along x, times 1/N^2), rho = |psi|^2,
+// forward along x, keep kx >= 0 retained panels (rho is real).
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) k_p1(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                double scale)
+{
+    VK_ROW_SETUP(C)
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double a = scale * v[s].x, b = scale * v[s].y;   // conj does not change |.|
+        v[s] = mk(a * a + b * b, 0.0);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        if (pp < g.Pp) Hrho[hoff<C>(pp, y, c)] = v[s];
+    }
+}
+
+// P2 (column, kx >= 0 retained panels): forward along y, dealias
+// (P_G), inverse along y, in place.  flags & 1: also the particle
+// potential column HV = IFFT_y(KV); flags & 2: store the dealiased
+// rho-hat row into K-space Krho (for the particle force).
+template <class C>
+__global__ void __launch_bounds__(C::COLT) k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
+                                                const cplx* __restrict__ KV, cplx* __restrict__ Krho,
+                                                const cplx* __restrict__ tw, Geo g, int flags)
+{
+    VK_COL_SETUP(C, blockIdx.x)
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = Hrho[hoff<C>(p, t + T * s, c)];
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int ky = t + T * s;
+        const bool m = kmasked(kx, ksg(ky, N), N);
+        if (m) v[s] = czero();
+        else if (flags & 2) Krho[(size_t)kx * N + ky] = v[s];
+        v[s] = conjc(v[s]);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) Hrho[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
+    if (flags & 1) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ky = t + T * s;
+            v[s] = kmasked(kx, ksg(ky, N), N) ? czero() : conjc(KV[(size_t)kx * N + ky]);
+        }
+        fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+        for (int s = 0; s < S; ++s) HV[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
+    }
+}
+
+// P3 (row): rho_d (+ i V) row from the kx >= 0 halves (Hermitian), psi row
+// from Hpsi, N = (gs*rho_d + vs*V) * psi, forward along x -> HN.
+// gs = +-g/N^2, vs = +-1/N^2 (RK4 +, ARGLE -); psi scaled by 1/N^2.
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) k_p3(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+                                                const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
+                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                int store_pmode, double gs, double vs, double scale, int flags)
+{
+    VK_ROW_SETUP(C)
+    cplx v[S];
+    const int Kp = TC * g.Pp;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int kxi = t + T * s;
+        cplx a = czero(), b = czero();
+        if (kxi < Kp) {
+            const size_t o = hoff<C>(kxi / TC, y, kxi % TC);
+            a = Hrhod[o];
+            if (flags & 1) b = HV[o];
+        } else if (kxi > N - Kp) {
+            const int km = N - kxi;
+            const size_t o = hoff<C>(km / TC, y, km % TC);
+            a = conjc(Hrhod[o]);
+            if (flags & 1) b = conjc(HV[o]);
+        }
+        v[s] = conjc(mk(a.x - b.y, a.y + b.x));   // a + i b, conj for the inverse
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    double f[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) f[s] = gs * v[s].x + ((flags & 1) ? vs * (-v[s].y) : 0.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const cplx ps = rmul(scale, conjc(v[s]));
+        v[s] = rmul(f[s], ps);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        if (panel_stored(pp, NP, g, store_pmode)) HN[hoff<C>(pp, y, c)] = v[s];
+    }
+}
+
+struct RKArgs {
+    cplx pref;      // (1 - i disp)/i
+    double alpha, omega;
+    double wacc;    // dt/6 (stages 1,4) or dt/6*2 (stages 2,3)
+    double cstep;   // dt/2, dt/2, dt, - for stages 1..3
+    int stage;
+};
+
+// P4 (column, retained panels both signs): forward along y -> N-hat;
+// RHS = pref*((alpha k^2 - omega) psi_s-hat + N-hat) on the retained disk;
+// RK4 accumulation and stage update (src/gpe.f90:366-487);
+// inverse along y of the new kpsi -> Hpsi.
+// K0 = str1 (= kpsi at step start), K1 = str2, K2 = stage kpsi.
+template <class C>
+__global__ void __launch_bounds__(C::COLT) k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+                                                cplx* __restrict__ K0, cplx* __restrict__ K1,
+                                                cplx* __restrict__ K2, const cplx* __restrict__ tw,
+                                                Geo g, RKArgs a)
+{
+    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, 1))
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    const int kxs = ksg(kx, N);
+    const size_t rowo = (size_t)kx * N;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int ky = t + T * s;
+        const int kys = ksg(ky, N);
+        cplx kn = czero();
+        if (!kmasked(kxs, kys, N)) {
+            const size_t o = rowo + ky;
+            const double kr2 = (double)(kxs * kxs + kys * kys);
+            const cplx s1 = K0[o];
+            const cplx ps = a.stage == 1 ? s1 : K2[o];
+            const cplx lin = rmul(a.alpha * kr2 - a.omega, ps);
+            const cplx rhs = cmul(a.pref, cadd(lin, v[s]));
+            const cplx inc = rmul(a.wacc, rhs);
+            const cplx acc = a.stage == 1 ? inc : cadd(K1[o], inc);
+            if (a.stage < 4) {
+                K1[o] = acc;
+                kn = cadd(s1, rmul(a.cstep, rhs));
+                K2[o] = kn;
+            } else {
+                kn = cadd(s1, acc);
+                K0[o] = kn;
+            }
+        }
+        v[s] = conjc(kn);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
+}
+
+// deterministic workgroup sum of NV values per thread; result valid in
+// thread 0.  sh must hold NV * (blockDim/64) doubles.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&x)[NV], double* sh)
+{
+    if (blockDim.x < 64) {
+        // partial wave: no cross-lane shuffles over inactive lanes
+        const int nt = blockDim.x;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[i * nt + threadIdx.x] = x[i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double s = 0.0;
+                for (int k = 0; k < nt; ++k) s += sh[i * nt + k];
+                x[i] = s;
+            }
+        }
+        __syncthreads();
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x[i] += __shfl_xor(x[i], o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[i * nw + w] = x[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < nw; ++k) s += sh[i * nw + k];
+            x[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+struct ArgleArgs {
+    double alpha, omega, dt;
+    int want_res;
+};
+
+// ARGLE A4 (column, all panels): forward along y -> N-hat; implicit update
+// kpsi = (kpsi + dt*(N-hat + omega*kpsi))/(1 + alpha*k^2*dt) on every mode
+// (src/gpe.f90:195-197, no dealiasing of the result); residual partial
+// sums; inverse along y of the new kpsi -> Hpsi.
+template <class C>
+__global__ void __launch_bounds__(C::COLT) k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+                                                cplx* __restrict__ K0, const cplx* __restrict__ tw,
+                                                ArgleArgs a, double* __restrict__ part)
+{
+    VK_COL_SETUP(C, blockIdx.x)
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    const int kxs = ksg(kx, N);
+    const size_t rowo = (size_t)kx * N;
+    double r[2] = {0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int ky = t + T * s;
+        const int kys = ksg(ky, N);
+        const size_t o = rowo + ky;
+        const cplx old = K0[o];
+        const double kr2 = (double)(kxs * kxs + kys * kys);
+        const cplx t1 = cadd(v[s], rmul(a.omega, old));
+        const double t2 = 1.0 + a.alpha * kr2 * a.dt;
+        const cplx num = cadd(old, rmul(a.dt, t1));
+        const cplx nw = mk(num.x / t2, num.y / t2);
+        K0[o] = nw;
+        if (a.want_res) {
+            const double dx = nw.x - old.x, dy = nw.y - old.y;
+            r[0] += dx * dx + dy * dy;
+            r[1] += old.x * old.x + old.y * old.y;
+        }
+        v[s] = conjc(nw);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
+    if (a.want_res) {
+        block_sum<2>(r, lds);
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = r[0];
+            part[2 * blockIdx.x + 1] = r[1];
+        }
+    }
+}
+
+// fixed-order reduction of nb partial records of nv doubles -> out[nv]
+__global__ void k_reduce_fixed(const double* __restrict__ part, int nb, int nv, double* __restrict__ out)
+{
+    __shared__ double sh[1024];
+    for (int i = 0; i < nv; ++i) {
+        double x = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) x += part[(size_t)b * nv + i];
+        double xx[1] = {x};
+        block_sum<1>(xx, sh);
+        if (threadIdx.x == 0) out[i] = xx[0];
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Diagnostics (src/diagnostics.f90:54-426)
+// ----------------------------------------------------------------------------
+
+struct DiagArgs {
+    double alpha, omega, g, smlnum;
+    int N;
+    int has_v;
+};
+
+// x-space pass: momentum, energies, norm partial sums (in the reference's
+// :84-207 quantities before the final factors) and the fields
+//   w = 2 alpha Im(psi* grad psi)/(|psi|+smlnum) -> W1, W2
+//   q = Re(psi* grad psi)/(|psi|+smlnum)          -> Q1, Q2   (:263-317)
+// sums: 0 momx 1 momy 2 egrad 3 echem 4 eintr 5 eppot 6 norm 7 unused
+__global__ void k_diag_x(const cplx* __restrict__ psi, const cplx* __restrict__ D1,
+                         const cplx* __restrict__ D2, const cplx* __restrict__ PR,
+                         const cplx* __restrict__ V, cplx* __restrict__ W1, cplx* __restrict__ W2,
+                         cplx* __restrict__ Q1, cplx* __restrict__ Q2, DiagArgs a, double* __restrict__ part)
+{
+    __shared__ double sh[8 * 16];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const size_t n2 = (size_t)a.N * a.N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        const cplx ps = psi[i], d1 = D1[i], d2 = D2[i], pr = PR[i];
+        const cplx cp = conjc(ps);
+        const cplx a1 = cmul(cp, d1), a2 = cmul(cp, d2);
+        const double apsi = hypot(ps.x, ps.y);
+        const double ap2 = apsi * apsi;
+        acc[0] += a1.y;
+        acc[1] += a2.y;
+        const double h1 = hypot(d1.x, d1.y), h2 = hypot(d2.x, d2.y);
+        acc[2] += a.alpha * (h1 * h1 + h2 * h2);
+        acc[3] += a.omega * ap2;
+        acc[4] += 0.5 * a.g * hypot(pr.x, pr.y) * ap2;
+        if (a.has_v) acc[5] += V[i].x * pr.x;
+        acc[6] += ap2;
+        const double rl = apsi + a.smlnum;
+        W1[i] = mk(2.0 * a.alpha * (a1.y / rl), 0.0);
+        W2[i] = mk(2.0 * a.alpha * (a2.y / rl), 0.0);
+        Q1[i] = mk(a1.x / rl, 0.0);
+        Q2[i] = mk(a2.x / rl, 0.0);
+    }
+    block_sum<8>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) part[(size_t)blockIdx.x * 8 + k] = acc[k];
+}
+
+struct SpecArgs {
+    double alpha, g, enorm;
+    int N, nshell;
+};
+
+// Shell binning: one workgroup per shell m over the list of retained modes
+// in that shell (idx = kx*N + ky), fixed-order reduction.
+// spec[(i)*(nshell+1) + m], i = 0..5 = spec_gp(m, i+1) (:394-399).
+__global__ void k_spectra(const int* __restrict__ modes, const int* __restrict__ offs,
+                          const cplx* __restrict__ KW1, const cplx* __restrict__ KW2,
+                          const cplx* __restrict__ KQ1, const cplx* __restrict__ KQ2,
+                          const cplx* __restrict__ KR, const cplx* __restrict__ KP, SpecArgs a,
+                          double* __restrict__ spec)
+{
+    __shared__ double sh[6 * 16];
+    const int m = blockIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = offs[m] + threadIdx.x; j < offs[m + 1]; j += blockDim.x) {
+        const int idx = modes[j];
+        const int kx = ksg(idx / a.N, a.N), ky = ksg(idx % a.N, a.N);
+        const double kxd = kx, kyd = ky, kr2 = kxd * kxd + kyd * kyd;
+        const cplx aa1 = KW1[idx], aa2 = KW2[idx];
+        cplx bb = czero();
+        if (!(kx == 0 && ky == 0)) {
+            // bb = -i (kx/k^2) aa1 - i (ky/k^2) aa2
+            const double f1 = kxd / kr2, f2 = kyd / kr2;
+            const cplx b1 = mk(f1 * aa1.y, -f1 * aa1.x), b2 = mk(f2 * aa2.y, -f2 * aa2.x);
+            bb = cadd(b1, b2);
+        }
+        const cplx cc1 = mk(-kxd * bb.y, kxd * bb.x), cc2 = mk(-kyd * bb.y, kyd * bb.x);
+        const cplx dd1 = csub(aa1, cc1), dd2 = csub(aa2, cc2);
+        const cplx e1 = KQ1[idx], e2 = KQ2[idx], ff = KR[idx], gg = KP[idx];
+        auto ab2 = [](cplx z) { const double h = hypot(z.x, z.y); return h * h; };
+        acc[0] += a.enorm * (ab2(aa1) + ab2(aa2)) * 0.5;
+        acc[1] += a.enorm * (ab2(dd1) + ab2(dd2)) * 0.5;
+        acc[2] += a.enorm * (ab2(cc1) + ab2(cc2)) * 0.5;
+        acc[3] += a.enorm * (ab2(e1) + ab2(e2)) * 2.0 * (a.alpha * a.alpha);
+        acc[4] += a.enorm * ab2(ff) * 0.5 * a.g * 2.0 * a.alpha;
+        acc[5] += a.enorm * ab2(gg);
+    }
+    block_sum<6>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 6; ++i) spec[(size_t)i * (a.nshell + 1) + m] = acc[i];
+}
+
+// ----------------------------------------------------------------------------
+// Particles (src/gpe_particle.f90)
+// ----------------------------------------------------------------------------
+
+// V(x,y) = V0 exp(-r^2/(2 d^2)), r from the periodic coordinate
+// (1/c)(s + s^3/6 + s^5/40), s = sin(c x), c = pi/L  (:34-56)
+__global__ void k_pot_x(cplx* __restrict__ X, int N, double dx, double c1, double V0, double d)
+{
+    const size_t n2 = (size_t)N * N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(i % N), iy = (int)(i / N);
+        const double x = ix * dx, y = iy * dx;
+        const double sx = sin(c1 * x), sy = sin(c1 * y);
+        const double sx2 = sx * sx, sy2 = sy * sy;
+        const double xp = (1.0 / c1) * (sx + (1.0 / 6.0) * (sx2 * sx) + (1.0 / 40.0) * (sx2 * sx2 * sx));
+        const double yp = (1.0 / c1) * (sy + (1.0 / 6.0) * (sy2 * sy) + (1.0 / 40.0) * (sy2 * sy2 * sy));
+        const double rp = sqrt(xp * xp + yp * yp);
+        X[i] = mk(V0 * exp(-(rp * rp) / (2.0 * (d * d))), 0.0);
+    }
+}
+
+// EX[p*N + i] = exp(-i kx(i) qx_p), EY[p*N + i] = exp(-i ky(i) qy_p)
+__global__ void k_ptab(const double* __restrict__ pos, cplx* __restrict__ EX, cplx* __restrict__ EY,
+                       int np, int N)
+{
+    const int n = np * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int pp = i / N, k = i % N;
+        const double kk = (double)ksg(k, N);
+        double sx, cx, sy, cy;
+        sincos(kk * pos[2 * pp], &sx, &cx);
+        sincos(kk * pos[2 * pp + 1], &sy, &cy);
+        EX[i] = mk(cx, -sx);
+        EY[i] = mk(cy, -sy);
+    }
+}
+
+// KV(kx,ky) = Vhat(kx,ky) * sum_p EX_p(kx) EY_p(ky) on the unmasked modes of
+// rows kx in [0, nrows) (nrows = N: all rows)   (:334-349, separable)
+__global__ void k_kv(const cplx* __restrict__ Vhat, const cplx* __restrict__ EX, const cplx* __restrict__ EY,
+                     cplx* __restrict__ KV, int np, int N, int rows_pos)
+{
+    extern __shared__ double dsh[];
+    cplx* ex = reinterpret_cast<cplx*>(dsh);
+    const int kx = blockIdx.x;
+    for (int pp = threadIdx.x; pp < np; pp += blockDim.x) ex[pp] = EX[(size_t)pp * N + kx];
+    __syncthreads();
+    const int kxs = ksg(kx, N);
+    (void)rows_pos;
+    for (int ky = threadIdx.x; ky < N; ky += blockDim.x) {
+        const int kys = ksg(ky, N);
+        if (kmasked(kxs, kys, N)) continue;
+        cplx sacc = czero();
+        for (int pp = 0; pp < np; ++pp) sacc = cadd(sacc, cmul(ex[pp], EY[(size_t)pp * N + ky]));
+        const size_t o = (size_t)kx * N + ky;
+        KV[o] = cmul(sacc, Vhat[o]);
+    }
+}
+
+// Force partial sums for rows kx: for each particle p (PG particles per
+// block in blockIdx.y), B_p = sum_ky A conj(EY_p(ky)), By_p = sum_ky ky A
+// conj(EY_p(ky)) with A = rhohat_d(k) * conj(Vhat(k)) = rhohat_d(k) Vhat(-k)
+// (V real).  out: fpart[((kx*np)+p)*4 + {ReB, ImB, ReBy, ImBy}]
+template <int PG>
+__global__ void k_force_rows(const cplx* __restrict__ Krho, const cplx* __restrict__ Vhat,
+                             const cplx* __restrict__ EY, double* __restrict__ fpart, int np, int N)
+{
+    __shared__ double sh[4 * PG * 4];
+    const int kx = blockIdx.x, p0 = blockIdx.y * PG;
+    const int kxs = ksg(kx, N);
+    double acc[4 * PG];
+#pragma unroll
+    for (int i = 0; i < 4 * PG; ++i) acc[i] = 0.0;
+    for (int ky = threadIdx.x; ky < N; ky += blockDim.x) {
+        const int kys = ksg(ky, N);
+        if (kmasked(kxs, kys, N)) continue;
+        const size_t o = (size_t)kx * N + ky;
+        const cplx A = cmul(Krho[o], conjc(Vhat[o]));
+#pragma unroll
+        for (int j = 0; j < PG; ++j) {
+            const int pp = p0 + j;
+            if (pp < np) {
+                const cplx z = cmul(A, conjc(EY[(size_t)pp * N + ky]));
+                acc[4 * j + 0] += z.x;
+                acc[4 * j + 1] += z.y;
+                acc[4 * j + 2] += (double)kys * z.x;
+                acc[4 * j + 3] += (double)kys * z.y;
+            }
+        }
+    }
+    block_sum<4 * PG>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < PG; ++j)
+            if (p0 + j < np)
+                for (int k = 0; k < 4; ++k) fpart[((size_t)kx * np + p0 + j) * 4 + k] = acc[4 * j + k];
+}
+
+struct PartArgs {
+    int np, N, nrows, half;   // rows kx in [0,nrows); half: weight 2 for kx>0
+    double coef;              // 2 alpha dvol / N^2
+    double rf;                // short-range factor 384 dE pi^12 r0^12 / L^12
+    double dt, mass;
+    int stage;                // 1..4 RK4 stage; 0: forces only (diagnostics)
+    double efac;              // 64 dE pi^12 r0^12 / L^12 (diagnostic energy)
+};
+
+// Particle forces and one RK4 stage of gp_particle_motion_rk4
+// (src/gpe_particle.f90:219-324, 378-531), one workgroup of 64 lanes.
+// Fluid force F_j = -2 alpha dvol Re sum_x d_j rho V_p
+//   = (2 alpha dvol/N^2) Im sum_k k_j rhohat_d(k) Vhat(-k) e^{i k.q_p}.
+// Particles advance one at a time (the short-range force on particle ip
+// sees the already advanced particles < ip, as in the reference).
+// st: pos(2np) vel(2np) fext(2np) tpos tvel ku[4](2np) kq[4](2np) frc(2np)
+//     then diag[5] = {eobj_kin, esrep, momx, momy, -}
+__global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict__ EX, double* __restrict__ st,
+                        PartArgs a)
+{
+    __shared__ double fl[2 * 256];
+    __shared__ double red[2];
+    const int np = a.np;
+    double* pos = st;
+    double* vel = st + 2 * np;
+    double* fext = st + 4 * np;
+    double* tpos = st + 6 * np;
+    double* tvel = st + 8 * np;
+    double* ku = st + 10 * np;   // [4][2np]
+    double* kq = st + 18 * np;   // [4][2np]
+    double* frc = st + 26 * np;
+    double* dg = st + 28 * np;
+    const int lane = threadIdx.x;
+    // fluid forces, fixed order over rows
+    for (int pp = lane; pp < np; pp += 64) {
+        double bx = 0.0, by = 0.0;
+        for (int kx = 0; kx < a.nrows; ++kx) {
+            const double* f = fpart + ((size_t)kx * np + pp) * 4;
+            const double w = (a.half && kx > 0) ? 2.0 : 1.0;
+            const cplx e = conjc(EX[(size_t)pp * a.N + kx]);   // e^{+i kx qx}
+            const double kxd = (double)ksg(kx, a.N);
+            const cplx B = cmul(e, mk(f[0], f[1]));
+            const cplx By = cmul(e, mk(f[2], f[3]));
+            bx += w * kxd * B.y;
+            by += w * By.y;
+        }
+        fl[2 * pp] = a.coef * bx;
+        fl[2 * pp + 1] = a.coef * by;
+    }
+    __syncthreads();
+    if (a.stage == 1)
+        for (int i = lane; i < 2 * np; i += 64) { tpos[i] = pos[i]; tvel[i] = vel[i]; }
+    __syncthreads();
+    for (int ip = 0; ip < np; ++ip) {
+        double sx = 0.0, sy = 0.0;
+        const double px = pos[2 * ip], py = pos[2 * ip + 1];
+        for (int qq = lane; qq < np; qq += 64) {
+            if (qq == ip) continue;
+            const double x = px - pos[2 * qq], y = py - pos[2 * qq + 1];
+            const double b = -3.0 + cos(x) + cos(y);
+            const double b2 = b * b, b4 = b2 * b2;
+            const double den = b4 * b2 * b;
+            sx += -a.rf * sin(x) / den;
+            sy += -a.rf * sin(y) / den;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+        }
+        if (np == 1) { sx = 0.0; sy = 0.0; }
+        if (lane == 0) {
+            const double F[2] = {fl[2 * ip] + sx + fext[2 * ip], fl[2 * ip + 1] + sy + fext[2 * ip + 1]};
+            frc[2 * ip] = F[0];
+            frc[2 * ip + 1] = F[1];
+            if (a.stage > 0) {
+                const int sidx = a.stage - 1;
+                for (int d = 0; d < 2; ++d) {
+                    const int i = 2 * ip + d;
+                    const double kuv = a.dt * F[d] / a.mass;
+                    const double kqv = a.dt * vel[i];
+                    ku[sidx * 2 * np + i] = kuv;
+                    kq[sidx * 2 * np + i] = kqv;
+                    if (a.stage <= 2) {
+                        vel[i] = tvel[i] + 0.5 * kuv;
+                        pos[i] = tpos[i] + 0.5 * kqv;
+                    } else if (a.stage == 3) {
+                        vel[i] = tvel[i] + kuv;
+                        pos[i] = tpos[i] + kqv;
+                    } else {
+                        vel[i] = tvel[i] + (1.0 / 6.0) * (ku[i] + 2.0 * ku[2 * np + i] + 2.0 * ku[4 * np + i] + ku[6 * np + i]);
+                        pos[i] = tpos[i] + (1.0 / 6.0) * (kq[i] + 2.0 * kq[2 * np + i] + 2.0 * kq[4 * np + i] + kq[6 * np + i]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (a.stage == 0) {
+        // particle energies and momentum (src/diagnostics.f90:210-239)
+        double ek = 0.0, es = 0.0, mx = 0.0, my = 0.0;
+        for (int i1 = lane; i1 < np; i1 += 64) {
+            ek += vel[2 * i1] * vel[2 * i1] + vel[2 * i1 + 1] * vel[2 * i1 + 1];
+            mx += a.mass * vel[2 * i1];
+            my += a.mass * vel[2 * i1 + 1];
+            for (int i2 = 0; i2 < np; ++i2) {
+                if (i1 == i2) continue;
+                const double x = pos[2 * i1] - pos[2 * i2], y = pos[2 * i1 + 1] - pos[2 * i2 + 1];
+                const double b = -3.0 + cos(x) + cos(y);
+                const double b2 = b * b;
+                es += a.efac / (b2 * b2 * b2);
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            ek += __shfl_xor(ek, o, 64);
+            es += __shfl_xor(es, o, 64);
+            mx += __shfl_xor(mx, o, 64);
+            my += __shfl_xor(my, o, 64);
+        }
+        if (lane == 0) {
+            dg[0] = 0.5 * a.mass * ek;
+            dg[1] = np == 1 ? 0.0 : 0.5 * es;
+            dg[2] = mx;
+            dg[3] = my;
+        }
+    }
+    (void)red;
+}
+
+// ----------------------------------------------------------------------------
+// Vortex initial condition
+// ----------------------------------------------------------------------------
+
+// Padé-core vortex product: amplitude prod_j sqrt(rho(r_j/xi)), r_j the
+// minimum-image distance; phase sum_j q_j sum_{|m|<=nimg}
+// arg(sin((x-x_j)/2 + i(y-y_j-2 pi m)/2)) - (sum_j q_j x_j) y/(2 pi), which
+// is 2 pi-periodic in x and y for zero net charge up to exp(-2 pi nimg).
+__host__ __device__ inline void ic_point(double x, double y, int nv, const double* xs, const double* ys,
+                                         const int* qs, double xi, int nimg, double* out)
+{
+    const double pi = 3.14159265358979323846;
+    const double a1 = 11.0 / 32.0, a2 = 11.0 / 384.0, b1 = 1.0 / 3.0;
+    double amp = 1.0, pr = 1.0, pim = 0.0, lin = 0.0;
+    for (int j = 0; j < nv; ++j) {
+        double dxp = x - xs[j], dyp = y - ys[j];
+        double wx = dxp - 2.0 * pi * floor(dxp / (2.0 * pi) + 0.5);
+        double wy = dyp - 2.0 * pi * floor(dyp / (2.0 * pi) + 0.5);
+        double r2 = (wx * wx + wy * wy) / (xi * xi);
+        double rho = r2 * (a1 + a2 * r2) / (1.0 + b1 * r2 + a2 * r2 * r2);
+        amp *= sqrt(rho);
+        lin += qs[j] * xs[j];
+        const double sX = sin(0.5 * dxp), cX = cos(0.5 * dxp);
+        for (int m = -nimg; m <= nimg; ++m) {
+            const double Y = 0.5 * (dyp - 2.0 * pi * m);
+            double zr = sX * cosh(Y), zi = cX * sinh(Y);
+            const double h = sqrt(zr * zr + zi * zi);
+            if (h > 0.0) { zr /= h; zi /= h; } else { zr = 1.0; zi = 0.0; }
+            if (qs[j] < 0) zi = -zi;
+            const double nr = pr * zr - pim * zi, ni = pr * zi + pim * zr;
+            pr = nr;
+            pim = ni;
+        }
+        const double hn = sqrt(pr * pr + pim * pim);
+        pr /= hn;
+        pim /= hn;
+    }
+    const double th = -lin * y / (2.0 * pi);
+    const double c = cos(th), s = sin(th);
+    out[0] = amp * (pr * c - pim * s);
+    out[1] = amp * (pr * s + pim * c);
+}
+
+__global__ void k_ic(cplx* __restrict__ X, int N, int nv, const double* __restrict__ xs,
+                     const double* __restrict__ ys, const int* __restrict__ qs, double xi, int nimg)
+{
+    const size_t n2 = (size_t)N * N;
+    const double dx = 2.0 * 3.14159265358979323846 / N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        const int ix = (int)(i % N), iy = (int)(i / N);
+        double o[2];
+        ic_point(ix * dx, iy * dx, nv, xs, ys, qs, xi, nimg, o);
+        X[i] = mk(o[0], o[1]);
+    }
+}
+
+}  // namespace vk

@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's GPE module interface, over the C ABI.
+
+The reference (Fortran) drives its step through argument-less subroutines on
+module globals: gpe_rk4 (src/gpe.f90:310), ARGLE(time) (src/gpe.f90:5),
+diagnostics_gpe(spoint,jtime,time) (src/diagnostics.f90:4),
+fft(plan_name,direction) (src/mod_fft.f90:471), particle_potential /
+initialize_particles (src/gpe_particle.f90:3,92).  Here one `GPE` object
+holds the device-resident state (psi, kpsi, particles) and exposes the same
+operations by the same names.  All compute runs in the HIP library; there is
+no CPU path.
+"""
+import math
+
+import numpy as np
+
+from . import _native as nat
+
+forward, inverse = -1, 1   # src/mod_fft.f90:48
+
+
+def params_from_namelist(Nx, dt_fix, xi, csound=None, rho_avg=1.0, disp_factor=0.0,
+                         gp_N_obj=0, ppotV0=0.0, ppotd=0.0, gp_mass_part=1.0,
+                         force_dealias=1):
+    """Build vk_gpe_params from the reference's namelist inputs
+    (src/driver.f90:52-78): xi in units of dx, csound absolute, ppotV0 in
+    units of g, ppotd in units of xi.  Derived constants follow
+    src/driver.f90:360-393.  csound=None selects g = 1 (csound = sqrt(2)*xi)."""
+    p = nat.Params()
+    if csound is None:
+        csound = math.sqrt(2.0) * xi * (2.0 * math.pi / Nx)
+    nat.check(nat.lib().vk_gpe_params_init(p, int(Nx), float(dt_fix), float(xi), float(csound),
+                                           float(rho_avg), float(disp_factor)))
+    p.np = int(gp_N_obj)
+    p.V0 = float(ppotV0) * p.g
+    p.d = float(ppotd) * p.xi
+    p.mass = float(gp_mass_part)
+    p.force_dealias = int(force_dealias)
+    return p
+
+
+def nshell(N):
+    return nat.lib().vk_nshell(int(N))
+
+
+class GPE:
+    """One device-resident GPE state (the reference's module globals)."""
+
+    def __init__(self, params):
+        self.params = params
+        self.N = params.N
+        self.nshell = nshell(self.N)
+        h = nat.ctypes.c_void_p()
+        nat.check(nat.lib().vk_gpe_create(nat.ctypes.byref(h), nat.ctypes.byref(params)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            nat.lib().vk_gpe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- fields ------------------------------------------------------------
+    def set_psi(self, psi, sync=True):
+        """psi[y, x] complex128.  kpsi = FFT(psi); sync: dealias kpsi and
+        psi = IFFT(kpsi) (restart_from_argle_data + ic_otoc_gpe)."""
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert a.shape == (self.N, self.N)
+        nat.check(nat.lib().vk_gpe_set_psi(self.h, nat.ptr(a), 1 if sync else 0))
+
+    def set_kpsi(self, kpsi):
+        a = np.ascontiguousarray(kpsi, dtype=np.complex128)
+        assert a.shape == (self.N, self.N)
+        nat.check(nat.lib().vk_gpe_set_kpsi(self.h, nat.ptr(a)))
+
+    def set_ic_vortices(self, xs, ys, qs, nimg=3, sync=True):
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        ys = np.ascontiguousarray(ys, dtype=np.float64)
+        qs = np.ascontiguousarray(qs, dtype=np.int32)
+        nat.check(nat.lib().vk_gpe_set_ic_vortices(self.h, len(xs), nat.ptr(xs), nat.ptr(ys),
+                                                   qs.ctypes.data_as(nat.ip), int(nimg), 1 if sync else 0))
+
+    @property
+    def psi(self):
+        out = np.empty((self.N, self.N), dtype=np.complex128)
+        nat.check(nat.lib().vk_gpe_get_psi(self.h, nat.ptr(out), None))
+        return out
+
+    @property
+    def kpsi(self):
+        out = np.empty((self.N, self.N), dtype=np.complex128)
+        nat.check(nat.lib().vk_gpe_get_psi(self.h, None, nat.ptr(out)))
+        return out
+
+    # ---- the reference's operations ------------------------------------------
+    def gpe_rk4(self, nsteps=1):
+        """nsteps x gpe_rk4 (src/gpe.f90:310-494)."""
+        nat.check(nat.lib().vk_gpe_rk4_steps(self.h, int(nsteps)))
+
+    def ARGLE(self, time=None, nsteps=1):
+        """nsteps x ARGLE(time) (src/gpe.f90:5-224); returns the residual
+        ||kpsi_new - kpsi_old|| / (dt ||kpsi_old||) of the last step."""
+        r = np.zeros(1)
+        nat.check(nat.lib().vk_gpe_argle_steps(self.h, int(nsteps), nat.ptr(r)))
+        return float(r[0])
+
+    def diagnostics_gpe(self, spoint=0, jtime=0, time=0.0):
+        """diagnostics_gpe (src/diagnostics.f90:4-571): returns a dict with
+        egrad, echem, eintr, etot, norm, momx, momy, n0, spec (6, nshell+1)
+        and, with particles, eppot, eobj_kin, esrep, momobj, frc."""
+        scal = np.zeros(8)
+        spec = np.zeros(6 * (self.nshell + 1))
+        pscal = np.zeros(5)
+        nat.check(nat.lib().vk_gpe_diagnostics(self.h, nat.ptr(scal), nat.ptr(spec), nat.ptr(pscal)))
+        d = dict(zip(["egrad", "echem", "eintr", "etot", "norm", "momx", "momy", "n0"], scal.tolist()))
+        d["spec"] = spec.reshape(6, self.nshell + 1)
+        d["scal"] = scal
+        if self.params.np > 0:
+            d.update(eppot=pscal[0], eobj_kin=pscal[1], esrep=pscal[2], momobj=pscal[3:5].copy())
+            d["frc"] = self.particles()[2]
+        return d
+
+    def fft(self, a, direction):
+        """fft(plan_name, direction) conformance hook (src/mod_fft.f90:471-542)
+        on an explicit array: forward x-space [y,x] -> k-space [kx,ky];
+        inverse k-space -> x-space times 1/(Nx*Ny)."""
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        out = np.empty_like(a)
+        nat.check(nat.lib().vk_fft2d(self.h, int(direction), nat.ptr(a), nat.ptr(out)))
+        return out
+
+    # ---- particles -----------------------------------------------------------
+    def initialize_particles(self, pos, vel=None, fext=None):
+        """particle_potential + initialize_particles (src/gpe_particle.f90:3-141);
+        pos/vel/fext are (np, 2) arrays (x, y per particle)."""
+        n = self.params.np
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 2)
+        vel = np.zeros((n, 2)) if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 2)
+        fext = np.zeros((n, 2)) if fext is None else np.ascontiguousarray(fext, dtype=np.float64).reshape(n, 2)
+        nat.check(nat.lib().vk_gpe_particles_set(self.h, nat.ptr(pos), nat.ptr(vel), nat.ptr(fext)))
+
+    def particles(self):
+        n = self.params.np
+        pos, vel, frc = np.zeros((n, 2)), np.zeros((n, 2)), np.zeros((n, 2))
+        nat.check(nat.lib().vk_gpe_particles_get(self.h, nat.ptr(pos), nat.ptr(vel), nat.ptr(frc)))
+        return pos, vel, frc
+
+    # ---- timing ----------------------------------------------------------------
+    def rk4_async(self, nsteps):
+        nat.check(nat.lib().vk_gpe_rk4_steps_async(self.h, int(nsteps)))
+
+    @property
+    def stream(self):
+        return nat.lib().vk_gpe_stream(self.h)
+
+    def pass_times(self, nsteps):
+        k = nat.lib().vk_pass_kinds()
+        ms = np.zeros(k)
+        nat.check(nat.lib().vk_gpe_pass_times(self.h, int(nsteps), nat.ptr(ms), k))
+        names = [nat.lib().vk_pass_name(i).decode() for i in range(k)]
+        return dict(zip(names, ms.tolist()))
+
+    def pass_bytes(self):
+        k = nat.lib().vk_pass_kinds()
+        names = [nat.lib().vk_pass_name(i).decode() for i in range(k)]
+        return {names[i]: nat.lib().vk_gpe_pass_bytes(self.h, i) for i in range(k)}
+
+
+def ic_vortices_host(N, xs, ys, qs, xi, nimg=3):
+    """Padé-core vortex product on the CPU (small grids; input generation)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    qs = np.ascontiguousarray(qs, dtype=np.int32)
+    out = np.empty((N, N), dtype=np.complex128)
+    nat.check(nat.lib().vk_ic_vortices_host(int(N), len(xs), nat.ptr(xs), nat.ptr(ys),
+                                            qs.ctypes.data_as(nat.ip), float(xi), int(nimg), nat.ptr(out)))
+    return out
+
+
+def lattice_vortices(n=4, L=2.0 * math.pi):
+    """n x n lattice at ((i+1/2)L/n, (j+1/2)L/n), charge (-1)^(i+j)."""
+    xs, ys, qs = [], [], []
+    for i in range(n):
+        for j in range(n):
+            xs.append((i + 0.5) * L / n)
+            ys.append((j + 0.5) * L / n)
+            qs.append(1 if (i + j) % 2 == 0 else -1)
+    return np.array(xs), np.array(ys), np.array(qs, dtype=np.int32)
+
+
+def random_vortices(nv, seed, L=2.0 * math.pi):
+    """nv vortices at seeded uniform positions with zero net charge."""
+    rng = np.random.default_rng(seed)
+    xs = rng.uniform(0.0, L, nv)
+    ys = rng.uniform(0.0, L, nv)
+    qs = np.array([1] * (nv // 2) + [-1] * (nv - nv // 2), dtype=np.int32)
+    rng.shuffle(qs)
+    return xs, ys, qs

@@ -1,0 +1,167 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle and
+numpy on the same seeded inputs.  Tolerance: FP64 fields to relative L2
+<= 1e-11 (the bar the reference's FP64 path is held to)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-11
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def G():
+    import torch
+    assert torch.cuda.is_available()
+    from quantumvortices_amd import gpe
+    return gpe
+
+
+def rand_field(N, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+
+
+@pytest.mark.parametrize("N", [32, 64, 128, 256, 512, 1024, 2048, 4096])
+def test_fft2d_vs_numpy(G, N):
+    say("case fft2d N=%d" % N)
+    p = G.params_from_namelist(N, 1e-3, 2.5)
+    g = G.GPE(p)
+    a = rand_field(N, N)
+    k = g.fft(a, -1)
+    ref = np.fft.fft2(a).T          # k-space [kx, ky]
+    assert rel_l2(k, ref) < 1e-13
+    back = g.fft(k, +1)
+    assert rel_l2(back, a) < 1e-13
+    g.close()
+
+
+@pytest.mark.parametrize("N", [32, 128, 1024])
+def test_set_get_psi(G, N):
+    say("case set/get N=%d" % N)
+    p = G.params_from_namelist(N, 1e-3, 2.5)
+    g = G.GPE(p)
+    a = rand_field(N, 7)
+    g.set_psi(a, sync=False)
+    assert rel_l2(g.psi, a) < 1e-13
+    assert rel_l2(g.kpsi, np.fft.fft2(a).T) < 1e-13
+    g.close()
+
+
+def _ic(G, N, nv_lattice=4, xi_in=2.5):
+    p = G.params_from_namelist(N, 1e-3, xi_in)
+    xs, ys, qs = G.lattice_vortices(nv_lattice)
+    return G.ic_vortices_host(N, xs, ys, qs, p.xi)
+
+
+@pytest.mark.parametrize("N,steps", [(32, 20), (64, 20), (128, 100), (256, 10), (512, 4)])
+def test_rk4_vs_oracle(G, oracle_mod, N, steps):
+    say("case rk4 N=%d steps=%d" % (N, steps))
+    O = oracle_mod
+    dt = 1e-3
+    psi0 = _ic(G, N)
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5))
+    g.set_psi(psi0, sync=True)
+    g.gpe_rk4(steps)
+    st = O.State(O.make_params(N, dt, 2.5))
+    st.set_psi(psi0, sync=True)
+    st.rk4(steps)
+    e1, e2 = rel_l2(g.psi, st.psi), rel_l2(g.kpsi, st.kpsi)
+    say("  rel-L2 psi %.3e kpsi %.3e" % (e1, e2))
+    assert e1 < TOL and e2 < TOL
+    g.close()
+
+
+@pytest.mark.parametrize("N,steps", [(64, 10), (128, 20)])
+def test_argle_vs_oracle(G, oracle_mod, N, steps):
+    say("case argle N=%d steps=%d" % (N, steps))
+    O = oracle_mod
+    dt = 1e-2
+    psi0 = _ic(G, N)
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5))
+    g.set_psi(psi0, sync=False)
+    r_gpu = g.ARGLE(nsteps=steps)
+    st = O.State(O.make_params(N, dt, 2.5))
+    st.set_psi(psi0, sync=False)
+    r_orc = st.argle(steps)
+    e = rel_l2(g.kpsi, st.kpsi)
+    say("  rel-L2 kpsi %.3e residual gpu %.6e oracle %.6e" % (e, r_gpu, r_orc))
+    assert e < TOL
+    assert abs(r_gpu - r_orc) <= 1e-9 * abs(r_orc)
+    g.close()
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_diagnostics_vs_oracle(G, oracle_mod, N):
+    say("case diagnostics N=%d" % N)
+    O = oracle_mod
+    dt = 1e-3
+    psi0 = _ic(G, N)
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5))
+    g.set_psi(psi0, sync=True)
+    g.gpe_rk4(3)
+    d = g.diagnostics_gpe()
+    st = O.State(O.make_params(N, dt, 2.5))
+    st.set_psi(psi0, sync=True)
+    st.rk4(3)
+    scal, spec, _ = st.diagnostics()
+    say("  scal gpu", d["scal"])
+    say("  scal orc", scal)
+    # energies and norm: relative; momentum is zero by the lattice's
+    # symmetry (roundoff-sized), so it is held to an absolute bound on the
+    # energy scale
+    for i in (0, 1, 2, 3, 4):
+        assert abs(d["scal"][i] - scal[i]) <= 1e-11 * abs(scal[i])
+    escale = abs(scal[0]) + abs(scal[1])
+    for i in (5, 6):
+        assert abs(d["scal"][i] - scal[i]) <= 1e-12 * escale
+    for i in range(6):
+        e = rel_l2(d["spec"][i], spec[i])
+        say("  spectrum %d rel-L2 %.3e" % (i + 1, e))
+        assert e < 1e-11
+    g.close()
+
+
+def test_particles_rk4_vs_oracle(G, oracle_mod):
+    N, dt, steps, npart = 64, 1e-3, 5, 3
+    say("case particles rk4 N=%d np=%d steps=%d" % (N, npart, steps))
+    O = oracle_mod
+    psi0 = _ic(G, N)
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(0, 2 * np.pi, (npart, 2))
+    vel = rng.uniform(-0.1, 0.1, (npart, 2))
+    fext = rng.uniform(-0.1, 0.1, (npart, 2))
+    kw = dict(gp_N_obj=npart, ppotV0=10.0, ppotd=2.0, gp_mass_part=5.0)
+    p = G.params_from_namelist(N, dt, 2.5, **kw)
+    g = G.GPE(p)
+    g.set_psi(psi0, sync=True)
+    g.initialize_particles(pos, vel, fext)
+    g.gpe_rk4(steps)
+    op = O.make_params(N, dt, 2.5, np_=npart, V0_in=10.0, d_in=2.0, mass=5.0)
+    st = O.State(op)
+    st.set_psi(psi0, sync=True)
+    st.part(0)[...] = pos
+    st.part(1)[...] = vel
+    st.part(2)[...] = fext
+    st.particle_potential()
+    st.rk4(steps)
+    gp, gv, _ = g.particles()
+    e = rel_l2(g.psi, st.psi)
+    say("  rel-L2 psi %.3e pos %.3e vel %.3e" % (e, rel_l2(gp, st.part(0)), rel_l2(gv, st.part(1))))
+    assert e < TOL
+    assert rel_l2(gp, st.part(0)) < TOL
+    assert rel_l2(gv, st.part(1)) < 1e-10
+    d = g.diagnostics_gpe()
+    scal, spec, pscal = st.diagnostics()
+    say("  frc gpu", d["frc"].ravel(), "orc", st.part(3).ravel())
+    assert rel_l2(d["frc"], st.part(3)) < 1e-10
+    assert abs(d["eppot"] - pscal[0]) <= 1e-11 * abs(pscal[0])
+    g.close()
