@@ -57,9 +57,13 @@ def test_set_get_psi(G, N):
 
 
 def _ic(G, N, nv_lattice=4, xi_in=2.5):
+    # lattice shifted off the grid points: a core exactly on a grid point
+    # makes |psi| roundoff-sized there and w = Im(psi* grad psi)/|psi|
+    # ill-conditioned for any implementation
     p = G.params_from_namelist(N, 1e-3, xi_in)
     xs, ys, qs = G.lattice_vortices(nv_lattice)
-    return G.ic_vortices_host(N, xs, ys, qs, p.xi)
+    dx = 2 * np.pi / N
+    return G.ic_vortices_host(N, xs + 0.31 * dx, ys + 0.17 * dx, qs, p.xi)
 
 
 @pytest.mark.parametrize("N,steps", [(32, 20), (64, 20), (128, 100), (256, 10), (512, 4)])
