@@ -8,7 +8,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libvikshobha.so")
+LIB_PATH = os.environ.get("VIKSHOBHA_LIB") or os.path.join(HERE, "lib", "libvikshobha.so")
 
 _lib = None
 

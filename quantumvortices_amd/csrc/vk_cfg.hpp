@@ -38,7 +38,19 @@ template <> struct Cfg<256> : CfgT<256, 16, 16, 16, true> {};
 template <> struct Cfg<512> : CfgT<512, 16, 8, 8, true> {};
 template <> struct Cfg<1024> : CfgT<1024, 16, 4, 4, true> {};
 template <> struct Cfg<2048> : CfgT<2048, 16, 4, 4, true> {};
-template <> struct Cfg<4096> : CfgT<4096, 16, 2, 2, true> {};
+#ifndef VK4096_S
+#define VK4096_S 16
+#endif
+#ifndef VK4096_TC
+#define VK4096_TC 2
+#endif
+#ifndef VK4096_TR
+#define VK4096_TR 2
+#endif
+#ifndef VK4096_SPLIT
+#define VK4096_SPLIT true
+#endif
+template <> struct Cfg<4096> : CfgT<4096, VK4096_S, VK4096_TC, VK4096_TR, VK4096_SPLIT> {};
 
 // Runtime panel geometry.  Retained |kx| <= M = floor(N/3) (the 2/3 rule
 // for 3 not dividing N).  Positive panels p < Pp cover kx in [0, M];

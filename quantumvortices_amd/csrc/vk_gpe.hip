@@ -75,6 +75,7 @@ struct vk_gpe {
     int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
     int kv_valid = 0;
     int have_part = 0;
+    int rs_part = 0;   // particle potential and force evaluated in real space inside P3
     // diagnostics
     cplx* X[7] = {};
     cplx* Kd[5] = {};
@@ -249,7 +250,7 @@ double ipow(double x, int n)
     return r;
 }
 
-vk::PartArgs part_args(const vk_gpe* h, int nrows, int half, int stage)
+vk::PartArgs part_args(const vk_gpe* h, int nrows, int half, int stage, int src = 0, int nblk = 0)
 {
     const double pi = 4.0 * std::atan(1.0);
     const double L = 2.0 * pi;
@@ -267,7 +268,37 @@ vk::PartArgs part_args(const vk_gpe* h, int nrows, int half, int stage)
     a.dt = h->p.dt;
     a.mass = h->p.mass;
     a.stage = stage;
+    a.src = src;
+    a.nblk = nblk;
+    a.coef_rs = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
     return a;
+}
+
+vk::P3Part p3part(const vk_gpe* h)
+{
+    vk::P3Part q{};
+    if (h->p.np > 0 && h->have_part) {
+        const double pi = 4.0 * std::atan(1.0);
+        q.pos = h->pst;
+        q.ppart = h->fpart;
+        q.np = h->p.np;
+        q.V0 = h->p.V0;
+        q.d = h->p.d;
+        q.W = 10.0 * h->p.d;   // exp(-50) of the peak at the window edge
+        q.dx = 2.0 * pi / h->N;
+    }
+    return q;
+}
+
+// Real-space particle path in P3: valid when the Gaussian's spectrum beyond
+// the dealias radius, exp(-(N/3 d)^2 / 2), is below 1e-20 of its peak (and
+// the aliasing beyond N/2 smaller still), so the reference's spectral
+// translation equals the analytic V(x - q) to far below FP64 roundoff.
+int use_real_space(const vk_gpe_params& p)
+{
+    if (p.np <= 0 || p.d <= 0.0) return 0;
+    const double kd = (p.N / 3.0) * p.d;
+    return 0.5 * kd * kd > 46.0;
 }
 
 // KV on rows [0, nrows) from the current positions
@@ -296,6 +327,20 @@ int particle_forces(vk_gpe* h, const cplx* Krho, int nrows, int half, int stage)
     return 0;
 }
 
+// particle stage update from the real-space block partials written by P3
+int particle_step_rs(vk_gpe* h, int nblk, int stage)
+{
+    ev_begin(h);
+    const double dx = 2.0 * (4.0 * std::atan(1.0)) / h->N;
+    const double coef = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
+    hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, h->stream, (const double*)h->fpart, nblk,
+                       h->p.np, coef, h->pst + 28 * h->p.np + 8);
+    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, h->stream, (const double*)h->fpart, (const cplx*)h->EX,
+                       h->pst, part_args(h, 0, 0, stage, 1, nblk));
+    ev_end(h, PK_PART);
+    return 0;
+}
+
 // ---- RK4 / ARGLE -----------------------------------------------------------------
 
 int rk4_step(vk_gpe* h)
@@ -305,19 +350,25 @@ int rk4_step(vk_gpe* h)
         const double sc = scale_of(h);
         const double dt = h->p.dt, oney6 = 1.0 / 6.0;
         const int parts = h->have_part && h->p.np > 0;
+        const int rs = parts && h->rs_part;
         const int Kp = C::TC * h->geo.Pp;
+        const vk::P3Part pa = p3part(h);
         vk::RKArgs a{};
         a.pref = make_double2(-h->p.disp, -1.0);   // (1 - i disp)/i
         a.alpha = h->p.alpha;
         a.omega = h->p.omega;
         for (int stage = 1; stage <= 4; ++stage) {
-            if (parts) particle_kv(h, Kp);
+            if (parts && !rs) particle_kv(h, Kp);
             row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc);
             col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
-                          (const cplx*)h->tw, h->geo, parts ? 3 : 0);
-            if (parts) particle_forces(h, h->Krho, Kp, 1, stage);
-            row_launch<C>(h, PK_P3, vk::k_p3<C>, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
-                          h->HN, (const cplx*)h->tw, h->geo, 1, 1, h->p.g * sc, sc, sc, parts ? 1 : 0);
+                          (const cplx*)h->tw, h->geo, (parts && !rs) ? 3 : 0);
+            if (parts && !rs) particle_forces(h, h->Krho, Kp, 1, stage);
+            {
+                auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 6> : vk::k_p3<C, 1>);
+                row_launch<C>(h, PK_P3, k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
+                              h->HN, (const cplx*)h->tw, h->geo, 1, 1, h->p.g * sc, sc, sc, 1.0, pa);
+            }
+            if (rs) particle_step_rs(h, C::N / C::TR, stage);
             a.stage = stage;
             a.wacc = (stage == 1 || stage == 4) ? dt * oney6 : dt * oney6 * 2.0;
             a.cstep = stage == 3 ? dt : dt * 0.5;
@@ -335,15 +386,20 @@ int argle_step(vk_gpe* h, int want_res)
         using C = CfgN<decltype(n)::value>;
         const double sc = scale_of(h);
         const int parts = h->have_part && h->p.np > 0;
-        if (parts && !h->kv_valid) {
+        const int rs = parts && h->rs_part;
+        const vk::P3Part pa = p3part(h);
+        if (parts && !rs && !h->kv_valid) {
             particle_kv(h, C::TC * h->geo.Pp);
             h->kv_valid = 1;
         }
         row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc);
         col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
-                      (const cplx*)h->tw, h->geo, parts ? 1 : 0);
-        row_launch<C>(h, PK_P3, vk::k_p3<C>, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi, h->HN,
-                      (const cplx*)h->tw, h->geo, 0, 0, -h->p.g * sc, -sc, sc, parts ? 1 : 0);
+                      (const cplx*)h->tw, h->geo, (parts && !rs) ? 1 : 0);
+        {
+            auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 2> : vk::k_p3<C, 1>);
+            row_launch<C>(h, PK_P3, k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi, h->HN,
+                          (const cplx*)h->tw, h->geo, 0, 0, -h->p.g * sc, -sc, sc, -1.0, pa);
+        }
         vk::ArgleArgs a{h->p.alpha, h->p.omega, h->p.dt, want_res};
         col_launch<C>(h, PK_ARG4, h->NP, vk::k_a4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, (const cplx*)h->tw, a,
                       h->part);
@@ -529,6 +585,7 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
     h->geo.M = M;
     h->geo.Pp = (M + 1 + TC - 1) / TC;
     h->geo.Pn = (M + TC - 1) / TC;
+    h->rs_part = use_real_space(*p);
     if (h->geo.Pp + h->geo.Pn > h->NP) { h->geo.Pp = h->NP; h->geo.Pn = 0; }
     auto cleanup = [&](int rc) { vk_gpe_destroy(h); return rc; };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
@@ -570,9 +627,9 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         if (hipMalloc(&h->EX, sizeof(cplx) * np * N) != hipSuccess ||
             hipMalloc(&h->EY, sizeof(cplx) * np * N) != hipSuccess ||
             hipMalloc(&h->fpart, sizeof(double) * 4 * np * N) != hipSuccess ||
-            hipMalloc(&h->pst, sizeof(double) * (28 * np + 8)) != hipSuccess)
+            hipMalloc(&h->pst, sizeof(double) * (30 * np + 8)) != hipSuccess)
             return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particle tables)"));
-        hipMemset(h->pst, 0, sizeof(double) * (28 * np + 8));
+        hipMemset(h->pst, 0, sizeof(double) * (30 * np + 8));
     } else {
         // P2/P3 take these pointers even when unused
         h->HV = h->Hrho;

@@ -6,11 +6,27 @@
 #pragma once
 #include "vk_cfg.hpp"
 
+// optional occupancy request for the pass kernels (waves per SIMD)
+#ifdef VK_WAVES
+#define VK_OCC __attribute__((amdgpu_waves_per_eu(VK_WAVES)))
+#else
+#define VK_OCC
+#endif
+
 namespace vk {
 
 // ----------------------------------------------------------------------------
 // thread <-> data mapping helpers
 // ----------------------------------------------------------------------------
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Row passes remap so that each XCD takes a contiguous run of row
+// blocks: neighbouring row blocks share 128-byte lines of the panel layout,
+// and those lines then stay in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
 
 #define VK_ROW_SETUP(C)                                                    \
     constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC, TR = C::TR;    \
@@ -21,7 +37,7 @@ namespace vk {
     const int q_ = u % (TR * TC);                                          \
     const int rr = q_ / TC, c = q_ % TC, pidx0 = u / (TR * TC);            \
     const int t = TC * pidx0 + c;                                          \
-    const int y = blockIdx.x * TR + rr;                                    \
+    const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + rr;             \
     double* lg = lds + rr * C::LDSW;
 
 #define VK_COL_SETUP(C, PEXPR)                                             \
@@ -46,7 +62,7 @@ __device__ __forceinline__ cplx czero() { return mk(0.0, 0.0); }
 
 // x-space rows -> FFT along x -> H.  op 1: transform |X|^2 instead of X.
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) k_row_fwd(const cplx* __restrict__ X, cplx* __restrict__ H,
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd(const cplx* __restrict__ X, cplx* __restrict__ H,
                                                      const cplx* __restrict__ tw, Geo g, int pstore, int op)
 {
     VK_ROW_SETUP(C)
@@ -67,7 +83,7 @@ __global__ void __launch_bounds__(C::ROWT) k_row_fwd(const cplx* __restrict__ X,
 
 // H column -> FFT along y -> K.  op 1: dealias.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
                                                      const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
     VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, pmode))
@@ -89,7 +105,7 @@ __global__ void __launch_bounds__(C::COLT) k_col_fwd(const cplx* __restrict__ H,
 // K row -> op -> unnormalised inverse FFT along y -> H.
 // op 0: as is; 1: dealias; 2: times i*ky.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
                                                      const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
     VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, pmode))
@@ -114,7 +130,7 @@ __global__ void __launch_bounds__(C::COLT) k_col_inv(const cplx* __restrict__ K,
 // pload 2: kx >= 0 panels stored, negative kx from Hermitian symmetry.
 // op 1: times i*kx before the transform.
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) k_row_inv(const cplx* __restrict__ H, cplx* __restrict__ X,
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv(const cplx* __restrict__ H, cplx* __restrict__ X,
                                                      const cplx* __restrict__ tw, Geo g, int pload, int op,
                                                      double scale)
 {
@@ -164,7 +180,7 @@ __global__ void k_dealias(cplx* __restrict__ K, int N)
 // P1 (row): psi row from Hpsi (inverse along x, times 1/N^2), rho = |psi|^2,
 // forward along x, keep kx >= 0 retained panels (rho is real).
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) k_p1(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
                                                 double scale)
 {
@@ -194,7 +210,7 @@ __global__ void __launch_bounds__(C::ROWT) k_p1(const cplx* __restrict__ Hpsi, c
 // potential column HV = IFFT_y(KV); flags & 2: store the dealiased
 // rho-hat row into K-space Krho (for the particle force).
 template <class C>
-__global__ void __launch_bounds__(C::COLT) k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
@@ -224,113 +240,6 @@ __global__ void __launch_bounds__(C::COLT) k_p2(cplx* __restrict__ Hrho, cplx* _
 #pragma unroll
         for (int s = 0; s < S; ++s) HV[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
     }
-}
-
-// P3 (row): rho_d (+ i V) row from the kx >= 0 halves (Hermitian), psi row
-// from Hpsi, N = (gs*rho_d + vs*V) * psi, forward along x -> HN.
-// gs = +-g/N^2, vs = +-1/N^2 (RK4 +, ARGLE -); psi scaled by 1/N^2.
-template <class C>
-__global__ void __launch_bounds__(C::ROWT) k_p3(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
-                                                const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
-                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
-                                                int store_pmode, double gs, double vs, double scale, int flags)
-{
-    VK_ROW_SETUP(C)
-    cplx v[S];
-    const int Kp = TC * g.Pp;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int kxi = t + T * s;
-        cplx a = czero(), b = czero();
-        if (kxi < Kp) {
-            const size_t o = hoff<C>(kxi / TC, y, kxi % TC);
-            a = Hrhod[o];
-            if (flags & 1) b = HV[o];
-        } else if (kxi > N - Kp) {
-            const int km = N - kxi;
-            const size_t o = hoff<C>(km / TC, y, km % TC);
-            a = conjc(Hrhod[o]);
-            if (flags & 1) b = conjc(HV[o]);
-        }
-        v[s] = conjc(mk(a.x - b.y, a.y + b.x));   // a + i b, conj for the inverse
-    }
-    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
-    double f[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) f[s] = gs * v[s].x + ((flags & 1) ? vs * (-v[s].y) : 0.0);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int pp = VK_PANEL(s);
-        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
-    }
-    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const cplx ps = rmul(scale, conjc(v[s]));
-        v[s] = rmul(f[s], ps);
-    }
-    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int pp = VK_PANEL(s);
-        if (panel_stored(pp, NP, g, store_pmode)) HN[hoff<C>(pp, y, c)] = v[s];
-    }
-}
-
-struct RKArgs {
-    cplx pref;      // (1 - i disp)/i
-    double alpha, omega;
-    double wacc;    // dt/6 (stages 1,4) or dt/6*2 (stages 2,3)
-    double cstep;   // dt/2, dt/2, dt, - for stages 1..3
-    int stage;
-};
-
-// P4 (column, retained panels both signs): forward along y -> N-hat;
-// RHS = pref*((alpha k^2 - omega) psi_s-hat + N-hat) on the retained disk;
-// RK4 accumulation and stage update (src/gpe.f90:366-487);
-// inverse along y of the new kpsi -> Hpsi.
-// K0 = str1 (= kpsi at step start), K1 = str2, K2 = stage kpsi.
-template <class C>
-__global__ void __launch_bounds__(C::COLT) k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
-                                                cplx* __restrict__ K0, cplx* __restrict__ K1,
-                                                cplx* __restrict__ K2, const cplx* __restrict__ tw,
-                                                Geo g, RKArgs a)
-{
-    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, 1))
-    cplx v[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
-    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
-    const int kxs = ksg(kx, N);
-    const size_t rowo = (size_t)kx * N;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int ky = t + T * s;
-        const int kys = ksg(ky, N);
-        cplx kn = czero();
-        if (!kmasked(kxs, kys, N)) {
-            const size_t o = rowo + ky;
-            const double kr2 = (double)(kxs * kxs + kys * kys);
-            const cplx s1 = K0[o];
-            const cplx ps = a.stage == 1 ? s1 : K2[o];
-            const cplx lin = rmul(a.alpha * kr2 - a.omega, ps);
-            const cplx rhs = cmul(a.pref, cadd(lin, v[s]));
-            const cplx inc = rmul(a.wacc, rhs);
-            const cplx acc = a.stage == 1 ? inc : cadd(K1[o], inc);
-            if (a.stage < 4) {
-                K1[o] = acc;
-                kn = cadd(s1, rmul(a.cstep, rhs));
-                K2[o] = kn;
-            } else {
-                kn = cadd(s1, acc);
-                K0[o] = kn;
-            }
-        }
-        v[s] = conjc(kn);
-    }
-    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
-#pragma unroll
-    for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
 }
 
 // deterministic workgroup sum of NV values per thread; result valid in
@@ -379,6 +288,196 @@ __device__ __forceinline__ void block_sum(double (&x)[NV], double* sh)
     __syncthreads();
 }
 
+// Real-space particle terms for P3 (used when the particle potential's
+// spectrum is negligible beyond the dealias radius, so that the reference's
+// spectral translation IFFT(exp(-i k.q) Vhat) equals V(x - q) to far below
+// FP64 roundoff; see the host-side selection).
+struct P3Part {
+    const double* pos;   // (x, y) per particle
+    double* ppart;       // per row block and particle: sum rho_raw * dV/dx, dV/dy
+    int np;
+    double V0, d, W, dx; // potential height, width, window radius, grid spacing
+};
+
+// minimum-image displacement in [-pi, pi)
+__device__ __forceinline__ double pwrap(double a)
+{
+    const double tp = 6.283185307179586476925;
+    return a - tp * rint(a / tp);
+}
+
+// periodic coordinate of src/gpe_particle.f90:46-50 (c = pi/L = 1/2):
+// X = (1/c)(s + s^3/6 + s^5/40), s = sin(c a); also dX/da
+__device__ __forceinline__ void pcoord(double a, double& X, double& dX)
+{
+    double sn, cs;
+    sincos(0.5 * a, &sn, &cs);
+    const double s2 = sn * sn;
+    X = 2.0 * (sn + (1.0 / 6.0) * (s2 * sn) + (1.0 / 40.0) * (s2 * s2 * sn));
+    dX = (1.0 + 0.5 * s2 + 0.125 * s2 * s2) * cs;
+}
+
+// P3 (row): rho_d (+ i V) row from the kx >= 0 halves (Hermitian), psi row
+// from Hpsi, N = (gs*rho_d + V) * psi, forward along x -> HN.
+// gs = +-g/N^2, vs = +-1/N^2 (RK4 +, ARGLE -); psi scaled by 1/N^2.
+// PM (particle mode, compile time): 0 none; 1 spectral potential (HV);
+// 2 real-space potential; 6 real-space potential + force partial sums.
+// vsign: +1 RK4, -1 ARGLE.
+template <class C, int PM>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+                                                const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
+                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                int store_pmode, double gs, double vs, double scale,
+                                                double vsign, P3Part pa)
+{
+    constexpr int flags = PM;
+    VK_ROW_SETUP(C)
+    cplx v[S];
+    const int Kp = TC * g.Pp;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int kxi = t + T * s;
+        cplx a = czero(), b = czero();
+        if (kxi < Kp) {
+            const size_t o = hoff<C>(kxi / TC, y, kxi % TC);
+            a = Hrhod[o];
+            if constexpr (flags & 1) b = HV[o];
+        } else if (kxi > N - Kp) {
+            const int km = N - kxi;
+            const size_t o = hoff<C>(km / TC, y, km % TC);
+            a = conjc(Hrhod[o]);
+            if constexpr (flags & 1) b = conjc(HV[o]);
+        }
+        v[s] = conjc(mk(a.x - b.y, a.y + b.x));   // a + i b, conj for the inverse
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    // v[s].x = rho_d (unnormalised); v[s].y = -V (unnormalised, flags & 1)
+    if constexpr ((flags & 2) != 0) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) v[s].y = 0.0;   // accumulates V(x - q) here
+        const int blk = xcd_remap(blockIdx.x, gridDim.x);
+        const int yb = blk * TR;
+        const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
+        for (int pp = 0; pp < pa.np; ++pp) {
+            const double qx = pa.pos[2 * pp], qy = pa.pos[2 * pp + 1];
+            bool hit = false;
+            for (int r2 = 0; r2 < TR; ++r2) hit |= fabs(pwrap((yb + r2) * pa.dx - qy)) <= pa.W;
+            if (!hit) {
+                if (((flags & 4) != 0) && threadIdx.x == 0) {
+                    pa.ppart[((size_t)blk * pa.np + pp) * 2] = 0.0;
+                    pa.ppart[((size_t)blk * pa.np + pp) * 2 + 1] = 0.0;
+                }
+                continue;
+            }
+            double fr[2] = {0.0, 0.0};
+            const double dyy = pwrap(y * pa.dx - qy);
+            if (fabs(dyy) <= pa.W) {
+                double Y, dY;
+                pcoord(dyy, Y, dY);
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const double dxx = pwrap((t + T * s) * pa.dx - qx);
+                    if (fabs(dxx) <= pa.W) {
+                        double X, dX;
+                        pcoord(dxx, X, dX);
+                        const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
+                        v[s].y += e;
+                        fr[0] += v[s].x * (-e * X * dX * invd2);
+                        fr[1] += v[s].x * (-e * Y * dY * invd2);
+                    }
+                }
+            }
+            if constexpr ((flags & 4) != 0) {
+                block_sum<2>(fr, lds);
+                if (threadIdx.x == 0) {
+                    pa.ppart[((size_t)blk * pa.np + pp) * 2] = fr[0];
+                    pa.ppart[((size_t)blk * pa.np + pp) * 2 + 1] = fr[1];
+                }
+            }
+        }
+    }
+    double f[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if constexpr ((flags & 1) != 0) f[s] = gs * v[s].x + vs * (-v[s].y);
+        else if constexpr ((flags & 2) != 0) f[s] = gs * v[s].x + vsign * v[s].y;
+        else f[s] = gs * v[s].x;
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const cplx ps = rmul(scale, conjc(v[s]));
+        v[s] = rmul(f[s], ps);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        if (panel_stored(pp, NP, g, store_pmode)) HN[hoff<C>(pp, y, c)] = v[s];
+    }
+}
+
+struct RKArgs {
+    cplx pref;      // (1 - i disp)/i
+    double alpha, omega;
+    double wacc;    // dt/6 (stages 1,4) or dt/6*2 (stages 2,3)
+    double cstep;   // dt/2, dt/2, dt, - for stages 1..3
+    int stage;
+};
+
+// P4 (column, retained panels both signs): forward along y -> N-hat;
+// RHS = pref*((alpha k^2 - omega) psi_s-hat + N-hat) on the retained disk;
+// RK4 accumulation and stage update (src/gpe.f90:366-487);
+// inverse along y of the new kpsi -> Hpsi.
+// K0 = str1 (= kpsi at step start), K1 = str2, K2 = stage kpsi.
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+                                                cplx* __restrict__ K0, cplx* __restrict__ K1,
+                                                cplx* __restrict__ K2, const cplx* __restrict__ tw,
+                                                Geo g, RKArgs a)
+{
+    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, 1))
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    const int kxs = ksg(kx, N);
+    const size_t rowo = (size_t)kx * N;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int ky = t + T * s;
+        const int kys = ksg(ky, N);
+        cplx kn = czero();
+        if (!kmasked(kxs, kys, N)) {
+            const size_t o = rowo + ky;
+            const double kr2 = (double)(kxs * kxs + kys * kys);
+            const cplx s1 = K0[o];
+            const cplx ps = a.stage == 1 ? s1 : K2[o];
+            const cplx lin = rmul(a.alpha * kr2 - a.omega, ps);
+            const cplx rhs = cmul(a.pref, cadd(lin, v[s]));
+            const cplx inc = rmul(a.wacc, rhs);
+            const cplx acc = a.stage == 1 ? inc : cadd(K1[o], inc);
+            if (a.stage < 4) {
+                K1[o] = acc;
+                kn = cadd(s1, rmul(a.cstep, rhs));
+                K2[o] = kn;
+            } else {
+                kn = cadd(s1, acc);
+                K0[o] = kn;
+            }
+        }
+        v[s] = conjc(kn);
+    }
+    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
+}
+
 struct ArgleArgs {
     double alpha, omega, dt;
     int want_res;
@@ -389,7 +488,7 @@ struct ArgleArgs {
 // (src/gpe.f90:195-197, no dealiasing of the result); residual partial
 // sums; inverse along y of the new kpsi -> Hpsi.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, const cplx* __restrict__ tw,
                                                 ArgleArgs a, double* __restrict__ part)
 {
@@ -636,7 +735,29 @@ __global__ void k_force_rows(const cplx* __restrict__ Krho, const cplx* __restri
                 for (int k = 0; k < 4; ++k) fpart[((size_t)kx * np + p0 + j) * 4 + k] = acc[4 * j + k];
 }
 
+// fl[p] = coef * sum over row blocks b (fixed order) of ppart[b][p] (x, y);
+// one workgroup per particle
+__global__ void k_pforce_rs(const double* __restrict__ ppart, int nblk, int np, double coef,
+                            double* __restrict__ fl)
+{
+    __shared__ double sh[2 * 16];
+    const int pp = blockIdx.x;
+    double acc[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        acc[0] += ppart[((size_t)b * np + pp) * 2];
+        acc[1] += ppart[((size_t)b * np + pp) * 2 + 1];
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0) {
+        fl[2 * pp] = coef * acc[0];
+        fl[2 * pp + 1] = coef * acc[1];
+    }
+}
+
 struct PartArgs {
+    int src;                  // 0: spectral row partials (fpart); 1: real-space block partials
+    int nblk;                 // real-space: number of row blocks
+    double coef_rs;           // real-space: 2 alpha dvol / N^2
     int np, N, nrows, half;   // rows kx in [0,nrows); half: weight 2 for kx>0
     double coef;              // 2 alpha dvol / N^2
     double rf;                // short-range factor 384 dE pi^12 r0^12 / L^12
@@ -669,8 +790,13 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
     double* frc = st + 26 * np;
     double* dg = st + 28 * np;
     const int lane = threadIdx.x;
-    // fluid forces, fixed order over rows
-    for (int pp = lane; pp < np; pp += 64) {
+    // fluid forces, fixed order over rows (spectral) or row blocks (real space)
+    for (int pp = lane; pp < np && a.src == 1; pp += 64) {
+        // reduced by k_pforce_rs into st[28 np + 8 ...]
+        fl[2 * pp] = st[28 * np + 8 + 2 * pp];
+        fl[2 * pp + 1] = st[28 * np + 8 + 2 * pp + 1];
+    }
+    for (int pp = lane; pp < np && a.src == 0; pp += 64) {
         double bx = 0.0, by = 0.0;
         for (int kx = 0; kx < a.nrows; ++kx) {
             const double* f = fpart + ((size_t)kx * np + pp) * 4;

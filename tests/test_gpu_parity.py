@@ -134,22 +134,25 @@ def test_diagnostics_vs_oracle(G, oracle_mod, N):
     g.close()
 
 
-def test_particles_rk4_vs_oracle(G, oracle_mod):
+# ppotd=2 (width 5 dx): potential and force evaluated in real space inside
+# the nonlinear pass; ppotd=1 (width 2.5 dx): spectral translation path
+@pytest.mark.parametrize("ppotd", [2.0, 1.0])
+def test_particles_rk4_vs_oracle(G, oracle_mod, ppotd):
     N, dt, steps, npart = 64, 1e-3, 5, 3
-    say("case particles rk4 N=%d np=%d steps=%d" % (N, npart, steps))
+    say("case particles rk4 N=%d np=%d steps=%d ppotd=%g" % (N, npart, steps, ppotd))
     O = oracle_mod
     psi0 = _ic(G, N)
     rng = np.random.default_rng(4)
     pos = rng.uniform(0, 2 * np.pi, (npart, 2))
     vel = rng.uniform(-0.1, 0.1, (npart, 2))
     fext = rng.uniform(-0.1, 0.1, (npart, 2))
-    kw = dict(gp_N_obj=npart, ppotV0=10.0, ppotd=2.0, gp_mass_part=5.0)
+    kw = dict(gp_N_obj=npart, ppotV0=10.0, ppotd=ppotd, gp_mass_part=5.0)
     p = G.params_from_namelist(N, dt, 2.5, **kw)
     g = G.GPE(p)
     g.set_psi(psi0, sync=True)
     g.initialize_particles(pos, vel, fext)
     g.gpe_rk4(steps)
-    op = O.make_params(N, dt, 2.5, np_=npart, V0_in=10.0, d_in=2.0, mass=5.0)
+    op = O.make_params(N, dt, 2.5, np_=npart, V0_in=10.0, d_in=ppotd, mass=5.0)
     st = O.State(op)
     st.set_psi(psi0, sync=True)
     st.part(0)[...] = pos
@@ -168,4 +171,28 @@ def test_particles_rk4_vs_oracle(G, oracle_mod):
     say("  frc gpu", d["frc"].ravel(), "orc", st.part(3).ravel())
     assert rel_l2(d["frc"], st.part(3)) < 1e-10
     assert abs(d["eppot"] - pscal[0]) <= 1e-11 * abs(pscal[0])
+    g.close()
+
+
+@pytest.mark.parametrize("ppotd", [2.0, 1.0])
+def test_particles_argle_vs_oracle(G, oracle_mod, ppotd):
+    N, dt, steps, npart = 64, 1e-2, 8, 2
+    say("case particles argle N=%d np=%d ppotd=%g" % (N, npart, ppotd))
+    O = oracle_mod
+    psi0 = _ic(G, N)
+    pos = np.array([[1.1, 2.3], [4.4, 0.7]])
+    p = G.params_from_namelist(N, dt, 2.5, gp_N_obj=npart, ppotV0=10.0, ppotd=ppotd, gp_mass_part=5.0)
+    g = G.GPE(p)
+    g.set_psi(psi0, sync=False)
+    g.initialize_particles(pos)
+    r = g.ARGLE(nsteps=steps)
+    st = O.State(O.make_params(N, dt, 2.5, np_=npart, V0_in=10.0, d_in=ppotd, mass=5.0))
+    st.set_psi(psi0, sync=False)
+    st.part(0)[...] = pos
+    st.particle_potential()
+    ro = st.argle(steps)
+    e = rel_l2(g.kpsi, st.kpsi)
+    say("  rel-L2 kpsi %.3e residual %.6e / %.6e" % (e, r, ro))
+    assert e < TOL
+    assert abs(r - ro) <= 1e-9 * abs(ro)
     g.close()
