@@ -26,6 +26,11 @@ struct CfgT {
     static constexpr int LDSW = Lds<N_>::words * (SPLIT_ ? 1 : 2);  // doubles per transform
     static constexpr int COLT = TC_ * T;  // column-pass threads
     static constexpr int ROWT = TR_ * T;  // row-pass threads
+    // LDS stride between the transforms of one workgroup: staggered by 16/G
+    // words so that transforms interleaved across lanes start on different
+    // bank pairs
+    static constexpr int COLRS = LDSW + 16 / TC_;
+    static constexpr int ROWRS = LDSW + 16 / TR_;
     static_assert(T % TC == 0, "TC must divide T");
     static_assert(T >= 1, "N/S >= 1");
 };

@@ -144,9 +144,9 @@ void set_lds(Kern k, size_t bytes)
 }
 
 template <class C, class Kern, class... A>
-void row_launch(vk_gpe* h, int kind, Kern k, A... args)
+void row_launch_x(vk_gpe* h, int kind, size_t extra, Kern k, A... args)
 {
-    const size_t lds = (size_t)C::TR * C::LDSW * sizeof(double);
+    const size_t lds = (size_t)C::TR * C::ROWRS * sizeof(double) + extra;
     set_lds(k, lds);
     ev_begin(h);
     hipLaunchKernelGGL(k, dim3(C::N / C::TR), dim3(C::ROWT), lds, h->stream, args...);
@@ -154,9 +154,21 @@ void row_launch(vk_gpe* h, int kind, Kern k, A... args)
 }
 
 template <class C, class Kern, class... A>
+void row_launch(vk_gpe* h, int kind, Kern k, A... args)
+{
+    row_launch_x<C>(h, kind, 0, k, args...);
+}
+
+// P3 keeps S doubles per thread in LDS besides the transform regions
+template <class C> size_t p3_extra(const vk_gpe* h)
+{
+    return (size_t)C::S * C::ROWT * sizeof(double) + (size_t)2 * std::max(h->p.np, 1) * sizeof(double);
+}
+
+template <class C, class Kern, class... A>
 void col_launch(vk_gpe* h, int kind, int nblocks, Kern k, A... args)
 {
-    const size_t lds = (size_t)C::TC * C::LDSW * sizeof(double);
+    const size_t lds = (size_t)C::TC * C::COLRS * sizeof(double);
     set_lds(k, lds);
     ev_begin(h);
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(C::COLT), lds, h->stream, args...);
@@ -365,7 +377,7 @@ int rk4_step(vk_gpe* h)
             if (parts && !rs) particle_forces(h, h->Krho, Kp, 1, stage);
             {
                 auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 6> : vk::k_p3<C, 1>);
-                row_launch<C>(h, PK_P3, k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
+                row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
                               h->HN, (const cplx*)h->tw, h->geo, 1, 1, h->p.g * sc, sc, sc, 1.0, pa);
             }
             if (rs) particle_step_rs(h, C::N / C::TR, stage);
@@ -397,7 +409,7 @@ int argle_step(vk_gpe* h, int want_res)
                       (const cplx*)h->tw, h->geo, (parts && !rs) ? 1 : 0);
         {
             auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 2> : vk::k_p3<C, 1>);
-            row_launch<C>(h, PK_P3, k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi, h->HN,
+            row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi, h->HN,
                           (const cplx*)h->tw, h->geo, 0, 0, -h->p.g * sc, -sc, sc, -1.0, pa);
         }
         vk::ArgleArgs a{h->p.alpha, h->p.omega, h->p.dt, want_res};

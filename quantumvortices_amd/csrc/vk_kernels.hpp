@@ -38,7 +38,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     const int rr = q_ / TC, c = q_ % TC, pidx0 = u / (TR * TC);            \
     const int t = TC * pidx0 + c;                                          \
     const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + rr;             \
-    double* lg = lds + rr * C::LDSW;
+    double* lg = lds + rr * C::ROWRS;
 
 #define VK_COL_SETUP(C, PEXPR)                                             \
     constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC;                \
@@ -49,7 +49,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     const int c = u % TC, t = u / TC;                                      \
     const int p = (PEXPR);                                                 \
     const int kx = p * TC + c;                                             \
-    double* lg = lds + c * C::LDSW;
+    double* lg = lds + c * C::COLRS;
 
 // panel holding kx = t + T*s in a row pass
 #define VK_PANEL(s) (pidx0 + (T / TC) * (s))
@@ -303,7 +303,7 @@ struct P3Part {
 __device__ __forceinline__ double pwrap(double a)
 {
     const double tp = 6.283185307179586476925;
-    return a - tp * rint(a / tp);
+    return a - tp * rint(a * (1.0 / tp));
 }
 
 // periodic coordinate of src/gpe_particle.f90:46-50 (c = pi/L = 1/2):
@@ -334,6 +334,12 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
+    // particle positions staged in LDS (after the f slots) during the first
+    // transform; read back as broadcasts in the particle loop
+    double* qlds = lds + TR * C::ROWRS + S * C::ROWT;
+    if constexpr ((flags & 2) != 0) {
+        for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
+    }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int kxi = t + T * s;
@@ -358,10 +364,18 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
         const int blk = xcd_remap(blockIdx.x, gridDim.x);
         const int yb = blk * TR;
         const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
+        const double idx = 1.0 / pa.dx, Wg = pa.W * idx, invN = 1.0 / N;
         for (int pp = 0; pp < pa.np; ++pp) {
-            const double qx = pa.pos[2 * pp], qy = pa.pos[2 * pp + 1];
+            const double qx = qlds[2 * pp], qy = qlds[2 * pp + 1];
+            const double cx = qx * idx, cy = qy * idx;   // centre in grid units
+            // window test in grid units (multiplications only): minimum-image
+            // row distance of the workgroup's rows to the particle
             bool hit = false;
-            for (int r2 = 0; r2 < TR; ++r2) hit |= fabs(pwrap((yb + r2) * pa.dx - qy)) <= pa.W;
+            for (int r2 = 0; r2 < TR; ++r2) {
+                double dr = (yb + r2) - cy;
+                dr -= N * rint(dr * invN);
+                hit |= fabs(dr) <= Wg;
+            }
             if (!hit) {
                 if (((flags & 4) != 0) && threadIdx.x == 0) {
                     pa.ppart[((size_t)blk * pa.np + pp) * 2] = 0.0;
@@ -370,16 +384,18 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
                 continue;
             }
             double fr[2] = {0.0, 0.0};
-            const double dyy = pwrap(y * pa.dx - qy);
-            if (fabs(dyy) <= pa.W) {
+            double dr = y - cy;
+            dr -= N * rint(dr * invN);
+            if (fabs(dr) <= Wg) {
                 double Y, dY;
-                pcoord(dyy, Y, dY);
+                pcoord(pwrap(y * pa.dx - qy), Y, dY);
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const double dxx = pwrap((t + T * s) * pa.dx - qx);
-                    if (fabs(dxx) <= pa.W) {
+                    double dc = (t + T * s) - cx;
+                    dc -= N * rint(dc * invN);
+                    if (fabs(dc) <= Wg) {
                         double X, dX;
-                        pcoord(dxx, X, dX);
+                        pcoord(pwrap((t + T * s) * pa.dx - qx), X, dX);
                         const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
                         v[s].y += e;
                         fr[0] += v[s].x * (-e * X * dX * invd2);
@@ -396,12 +412,16 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
             }
         }
     }
-    double f[S];
+    // the real factor f = g rho_d + V waits in thread-private LDS slots
+    // (after the transform regions) while psi is transformed
+    double* fl = lds + TR * C::ROWRS;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if constexpr ((flags & 1) != 0) f[s] = gs * v[s].x + vs * (-v[s].y);
-        else if constexpr ((flags & 2) != 0) f[s] = gs * v[s].x + vsign * v[s].y;
-        else f[s] = gs * v[s].x;
+        double fv;
+        if constexpr ((flags & 1) != 0) fv = gs * v[s].x + vs * (-v[s].y);
+        else if constexpr ((flags & 2) != 0) fv = gs * v[s].x + vsign * v[s].y;
+        else fv = gs * v[s].x;
+        fl[s * C::ROWT + threadIdx.x] = fv;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -412,7 +432,7 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const cplx ps = rmul(scale, conjc(v[s]));
-        v[s] = rmul(f[s], ps);
+        v[s] = rmul(fl[s * C::ROWT + threadIdx.x], ps);
     }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
 #pragma unroll
