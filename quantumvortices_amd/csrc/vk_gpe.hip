@@ -67,6 +67,8 @@ struct vk_gpe {
     Geo geo{};
     int NP = 0, TC = 0;
     hipStream_t stream = nullptr;
+    hipStream_t pstream = nullptr;     // particle update, overlapped with P4/P1/P2
+    hipEvent_t ev_p3 = nullptr, ev_ps = nullptr;
     cplx* tw = nullptr;
     // state and RK4/ARGLE work fields
     cplx *K0 = nullptr, *K1 = nullptr, *K2 = nullptr, *Hpsi = nullptr, *Hrho = nullptr, *HN = nullptr;
@@ -340,17 +342,32 @@ int particle_forces(vk_gpe* h, const cplx* Krho, int nrows, int half, int stage)
 }
 
 // particle stage update from the real-space block partials written by P3
+// The update depends on P3 (force partials) and is needed by the next
+// P3 (positions); it runs on a side stream while P4, P1, P2 proceed.
 int particle_step_rs(vk_gpe* h, int nblk, int stage)
 {
+    const bool side = !h->timing && h->pstream;
+    hipStream_t st = side ? h->pstream : h->stream;
+    if (side) {
+        hipEventRecord(h->ev_p3, h->stream);
+        hipStreamWaitEvent(h->pstream, h->ev_p3, 0);
+    }
     ev_begin(h);
     const double dx = 2.0 * (4.0 * std::atan(1.0)) / h->N;
     const double coef = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
-    hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, h->stream, (const double*)h->fpart, nblk,
+    hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, st, (const double*)h->fpart, nblk,
                        h->p.np, coef, h->pst + 28 * h->p.np + 8);
-    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, h->stream, (const double*)h->fpart, (const cplx*)h->EX,
+    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, st, (const double*)h->fpart, (const cplx*)h->EX,
                        h->pst, part_args(h, 0, 0, stage, 1, nblk));
     ev_end(h, PK_PART);
+    if (side) hipEventRecord(h->ev_ps, h->pstream);
     return 0;
+}
+
+// order the main stream after the last side-stream particle update
+void join_particles(vk_gpe* h)
+{
+    if (h->pstream && h->ev_ps) hipStreamWaitEvent(h->stream, h->ev_ps, 0);
 }
 
 // ---- RK4 / ARGLE -----------------------------------------------------------------
@@ -375,6 +392,7 @@ int rk4_step(vk_gpe* h)
             col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                           (const cplx*)h->tw, h->geo, (parts && !rs) ? 3 : 0);
             if (parts && !rs) particle_forces(h, h->Krho, Kp, 1, stage);
+            if (rs) join_particles(h);   // positions from the previous stage
             {
                 auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 6> : vk::k_p3<C, 1>);
                 row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
@@ -387,6 +405,7 @@ int rk4_step(vk_gpe* h)
             col_launch<C>(h, PK_P4, h->geo.Pp + h->geo.Pn, vk::k_p4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
                           h->K2, (const cplx*)h->tw, h->geo, a);
         }
+        if (rs) join_particles(h);
         h->kv_valid = 0;
         return 0;
     });
@@ -642,6 +661,10 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
             hipMalloc(&h->pst, sizeof(double) * (30 * np + 8)) != hipSuccess)
             return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particle tables)"));
         hipMemset(h->pst, 0, sizeof(double) * (30 * np + 8));
+        if (hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_p3, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_ps, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "particle stream/events"));
     } else {
         // P2/P3 take these pointers even when unused
         h->HV = h->Hrho;
@@ -674,6 +697,9 @@ void vk_gpe_destroy(vk_gpe* h)
     if (h->part) hipFree(h->part);
     if (h->dsum) hipFree(h->dsum);
     for (auto e : h->ev) hipEventDestroy(e);
+    if (h->pstream) { hipStreamSynchronize(h->pstream); hipStreamDestroy(h->pstream); }
+    if (h->ev_p3) hipEventDestroy(h->ev_p3);
+    if (h->ev_ps) hipEventDestroy(h->ev_ps);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }

@@ -357,10 +357,22 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
         v[s] = conjc(mk(a.x - b.y, a.y + b.x));   // a + i b, conj for the inverse
     }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
-    // v[s].x = rho_d (unnormalised); v[s].y = -V (unnormalised, flags & 1)
-    if constexpr ((flags & 2) != 0) {
+    // v[s].x = rho_d (unnormalised); v[s].y = -V (unnormalised, flags & 1).
+    // The real factor f = g rho_d + V goes to thread-private LDS slots (after
+    // the transform regions) and waits there while psi is transformed.
+    double* fl = lds + TR * C::ROWRS;
 #pragma unroll
-        for (int s = 0; s < S; ++s) v[s].y = 0.0;   // accumulates V(x - q) here
+    for (int s = 0; s < S; ++s) {
+        double fv;
+        if constexpr ((flags & 1) != 0) fv = gs * v[s].x + vs * (-v[s].y);
+        else fv = gs * v[s].x;
+        fl[s * C::ROWT + threadIdx.x] = fv;
+    }
+    if constexpr ((flags & 2) != 0) {
+        // real-space particle potential added into the slots; force partials
+        double rho[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) rho[s] = v[s].x;
         const int blk = xcd_remap(blockIdx.x, gridDim.x);
         const int yb = blk * TR;
         const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
@@ -397,9 +409,9 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
                         double X, dX;
                         pcoord(pwrap((t + T * s) * pa.dx - qx), X, dX);
                         const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
-                        v[s].y += e;
-                        fr[0] += v[s].x * (-e * X * dX * invd2);
-                        fr[1] += v[s].x * (-e * Y * dY * invd2);
+                        fl[s * C::ROWT + threadIdx.x] += vsign * e;
+                        fr[0] += rho[s] * (-e * X * dX * invd2);
+                        fr[1] += rho[s] * (-e * Y * dY * invd2);
                     }
                 }
             }
@@ -411,17 +423,6 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
                 }
             }
         }
-    }
-    // the real factor f = g rho_d + V waits in thread-private LDS slots
-    // (after the transform regions) while psi is transformed
-    double* fl = lds + TR * C::ROWRS;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        double fv;
-        if constexpr ((flags & 1) != 0) fv = gs * v[s].x + vs * (-v[s].y);
-        else if constexpr ((flags & 2) != 0) fv = gs * v[s].x + vsign * v[s].y;
-        else fv = gs * v[s].x;
-        fl[s * C::ROWT + threadIdx.x] = fv;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
