@@ -21,6 +21,25 @@ FLAGS = [
 ]
 
 
+FORTRAN = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+FSRC = [os.path.join(HERE, "fortran", "vikshobha_mod.f90"), os.path.join(HERE, "fortran", "driver_check.f90")]
+FBIN = os.path.join(HERE, "lib", "driver_check")
+
+
+def build_fortran(verbose=False):
+    """ISO_C_BINDING layer + the reference-style check driver (amdflang)."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod:
+        cmd = [FORTRAN, "-O2", "-J", mod, "-o", FBIN] + FSRC + [
+            "-L" + os.path.dirname(LIB), "-lvikshobha", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stderr)
+    return FBIN
+
+
 def build(verbose=False):
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     tmp = LIB + ".tmp"
@@ -32,6 +51,8 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + err)
     os.replace(tmp, LIB)
+    if os.path.exists(FORTRAN):
+        build_fortran(verbose)
     return LIB
 
 

@@ -1,0 +1,195 @@
+! ISO_C_BINDING layer over the vikshobha C ABI (include/vikshobha.h), keeping
+! the reference's module API so a Fortran driver can call it in place of
+! mod_fft / gpe / diagnostics:
+!   initialize_fft            <- src/mod_fft.f90:288 (plans) + create_state_variables :122
+!   fft(plan_name, direction) <- src/mod_fft.f90:471  (plan 'psi')
+!   gpe_rk4                   <- src/gpe.f90:310
+!   ARGLE(time)               <- src/gpe.f90:5
+!   diagnostics_gpe(spoint,jtime,time) <- src/diagnostics.f90:4
+!   clear_fft                 <- src/mod_fft.f90:433
+! psi(Nx,Ny) and kpsi(Ny,Nx) are host arrays with the reference's layouts.
+! The state lives on the device between calls; psi/kpsi are refreshed by
+! fft(...) and by sync_host().
+
+MODULE vikshobha_c
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+
+  TYPE, BIND(C) :: vk_gpe_params
+    INTEGER(C_INT) :: N
+    REAL(C_DOUBLE) :: dt, alpha, g, omega, disp, xi
+    INTEGER(C_INT) :: np
+    REAL(C_DOUBLE) :: V0, d, mass
+    INTEGER(C_INT) :: force_dealias
+  END TYPE
+
+  INTERFACE
+    INTEGER(C_INT) FUNCTION vk_gpe_params_init(p, N, dt, xi_in, csound, rho_avg, disp) BIND(C, NAME='vk_gpe_params_init')
+      IMPORT :: C_INT, C_DOUBLE, vk_gpe_params
+      TYPE(vk_gpe_params) :: p
+      INTEGER(C_INT), VALUE :: N
+      REAL(C_DOUBLE), VALUE :: dt, xi_in, csound, rho_avg, disp
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_nshell(N) BIND(C, NAME='vk_nshell')
+      IMPORT :: C_INT
+      INTEGER(C_INT), VALUE :: N
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_create(h, p) BIND(C, NAME='vk_gpe_create')
+      IMPORT :: C_INT, C_PTR, vk_gpe_params
+      TYPE(C_PTR) :: h
+      TYPE(vk_gpe_params) :: p
+    END FUNCTION
+    SUBROUTINE vk_gpe_destroy(h) BIND(C, NAME='vk_gpe_destroy')
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+    INTEGER(C_INT) FUNCTION vk_gpe_set_psi(h, psi, flags) BIND(C, NAME='vk_gpe_set_psi')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE_COMPLEX
+      TYPE(C_PTR), VALUE :: h
+      COMPLEX(C_DOUBLE_COMPLEX) :: psi(*)
+      INTEGER(C_INT), VALUE :: flags
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_set_kpsi(h, kpsi) BIND(C, NAME='vk_gpe_set_kpsi')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE_COMPLEX
+      TYPE(C_PTR), VALUE :: h
+      COMPLEX(C_DOUBLE_COMPLEX) :: kpsi(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_get_psi(h, psi, kpsi) BIND(C, NAME='vk_gpe_get_psi')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE_COMPLEX
+      TYPE(C_PTR), VALUE :: h
+      COMPLEX(C_DOUBLE_COMPLEX) :: psi(*), kpsi(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_rk4_steps(h, n) BIND(C, NAME='vk_gpe_rk4_steps')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: n
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_argle_steps(h, n, res) BIND(C, NAME='vk_gpe_argle_steps')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: n
+      REAL(C_DOUBLE) :: res
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_diagnostics(h, scal, spec, pscal) BIND(C, NAME='vk_gpe_diagnostics')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: scal(8), spec(*), pscal(5)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_ic_vortices_host(N, nv, xs, ys, qs, xi, nimg, psi) BIND(C, NAME='vk_ic_vortices_host')
+      IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX
+      INTEGER(C_INT), VALUE :: N, nv, nimg
+      REAL(C_DOUBLE) :: xs(*), ys(*)
+      INTEGER(C_INT) :: qs(*)
+      REAL(C_DOUBLE), VALUE :: xi
+      COMPLEX(C_DOUBLE_COMPLEX) :: psi(*)
+    END FUNCTION
+    FUNCTION vk_last_error() BIND(C, NAME='vk_last_error')
+      IMPORT :: C_PTR
+      TYPE(C_PTR) :: vk_last_error
+    END FUNCTION
+  END INTERFACE
+END MODULE vikshobha_c
+
+
+MODULE vikshobha_gpu
+  USE ISO_C_BINDING
+  USE vikshobha_c
+  IMPLICIT NONE
+
+  INTEGER, PARAMETER :: forward = -1, inverse = 1      ! src/mod_fft.f90:48
+  INTEGER :: Nx, Ny, nshell
+  TYPE(vk_gpe_params) :: params
+  TYPE(C_PTR) :: vkh = C_NULL_PTR
+  COMPLEX(C_DOUBLE_COMPLEX), ALLOCATABLE, TARGET :: psi(:,:), kpsi(:,:)
+  REAL(C_DOUBLE) :: egrad, echem, eintr, etot, norm, momx, momy, n0, argle_residual
+  REAL(C_DOUBLE), ALLOCATABLE :: spec_gp(:,:)
+
+CONTAINS
+
+  SUBROUTINE vk_check(rc, where)
+    INTEGER(C_INT), INTENT(IN) :: rc
+    CHARACTER(*), INTENT(IN) :: where
+    CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+    INTEGER :: i
+    IF (rc /= 0) THEN
+      CALL C_F_POINTER(vk_last_error(), msg, [512])
+      WRITE(*, '(A,A,I4,A)', ADVANCE='NO') where, ': vikshobha error ', rc, ': '
+      DO i = 1, 512
+        IF (msg(i) == C_NULL_CHAR) EXIT
+        WRITE(*, '(A)', ADVANCE='NO') msg(i)
+      END DO
+      WRITE(*, *)
+      STOP 2
+    END IF
+  END SUBROUTINE
+
+  ! set_grid + constants (src/mod_grid.f90:63-101, src/driver.f90:360-376)
+  ! followed by create_state_variables + initialize_fft
+  SUBROUTINE initialize_fft(N, dt_fix, xi, csound, rho_avg, disp_factor)
+    INTEGER, INTENT(IN) :: N
+    REAL(C_DOUBLE), INTENT(IN) :: dt_fix, xi, csound, rho_avg, disp_factor
+    CALL vk_check(vk_gpe_params_init(params, INT(N, C_INT), dt_fix, xi, csound, rho_avg, disp_factor), 'params_init')
+    CALL vk_check(vk_gpe_create(vkh, params), 'create')
+    Nx = N; Ny = N
+    nshell = vk_nshell(INT(N, C_INT))
+    ALLOCATE(psi(Nx, Ny), kpsi(Ny, Nx), spec_gp(0:nshell, 1:6))
+    psi = (0.0_C_DOUBLE, 0.0_C_DOUBLE); kpsi = psi
+  END SUBROUTINE
+
+  ! fft(plan_name, direction) for plan 'psi' (src/mod_fft.f90:482-487):
+  ! forward: kpsi = FFT(psi); inverse: psi = IFFT(kpsi)/(Nx*Ny)
+  SUBROUTINE fft(plan_name, direction)
+    CHARACTER(*), INTENT(IN) :: plan_name
+    INTEGER, INTENT(IN) :: direction
+    IF (TRIM(plan_name) /= 'psi') THEN
+      WRITE(*,*) 'fft: only plan psi is device resident in this layer'
+      STOP 2
+    END IF
+    IF (direction == forward) THEN
+      CALL vk_check(vk_gpe_set_psi(vkh, psi, 0_C_INT), 'fft forward')
+    ELSE
+      CALL vk_check(vk_gpe_set_kpsi(vkh, kpsi), 'fft inverse')
+    END IF
+    CALL sync_host()
+  END SUBROUTINE
+
+  ! restart_from_argle_data + ic_otoc_gpe: kpsi = FFT(psi), dealias, psi = IFFT(kpsi)
+  SUBROUTINE set_initial_psi()
+    CALL vk_check(vk_gpe_set_psi(vkh, psi, 1_C_INT), 'set_initial_psi')
+    CALL sync_host()
+  END SUBROUTINE
+
+  SUBROUTINE sync_host()
+    CALL vk_check(vk_gpe_get_psi(vkh, psi, kpsi), 'sync_host')
+  END SUBROUTINE
+
+  SUBROUTINE gpe_rk4()
+    CALL vk_check(vk_gpe_rk4_steps(vkh, 1_C_INT), 'gpe_rk4')
+  END SUBROUTINE
+
+  SUBROUTINE gpe_rk4_steps(n)
+    INTEGER, INTENT(IN) :: n
+    CALL vk_check(vk_gpe_rk4_steps(vkh, INT(n, C_INT)), 'gpe_rk4_steps')
+  END SUBROUTINE
+
+  SUBROUTINE ARGLE(time)
+    REAL(C_DOUBLE), INTENT(IN) :: time
+    CALL vk_check(vk_gpe_argle_steps(vkh, 1_C_INT, argle_residual), 'ARGLE')
+  END SUBROUTINE
+
+  SUBROUTINE diagnostics_gpe(spoint, jtime, time)
+    INTEGER, INTENT(IN) :: spoint, jtime
+    REAL(C_DOUBLE), INTENT(IN) :: time
+    REAL(C_DOUBLE) :: scal(8), pscal(5)
+    CALL vk_check(vk_gpe_diagnostics(vkh, scal, spec_gp, pscal), 'diagnostics_gpe')
+    egrad = scal(1); echem = scal(2); eintr = scal(3); etot = scal(4)
+    norm = scal(5); momx = scal(6); momy = scal(7); n0 = scal(8)
+  END SUBROUTINE
+
+  SUBROUTINE clear_fft()
+    IF (C_ASSOCIATED(vkh)) CALL vk_gpe_destroy(vkh)
+    vkh = C_NULL_PTR
+    IF (ALLOCATED(psi)) DEALLOCATE(psi, kpsi, spec_gp)
+  END SUBROUTINE
+
+END MODULE vikshobha_gpu
