@@ -122,6 +122,44 @@ def cpu_baseline(G, N, dt, wl, threads, psi_gpu):
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": "port", "sample": sample}
 
 
+def ensemble_bench(G, torch, steps, warmup):
+    """Batched ensemble: 384 independent realizations, 128 each at 128^2,
+    256^2 and 512^2, with 8-64 zero-net-charge random vortices (seed =
+    realization id); RK4 dt=1e-3; schedule picked by the autotuner, one
+    step replayed as a HIP graph."""
+    import numpy as np
+    sizes = [128] * 128 + [256] * 128 + [512] * 128
+    ps = [G.params_from_namelist(N, 1e-3, 2.5) for N in sizes]
+    e = G.Ensemble(ps)
+    for r in range(len(sizes)):
+        rng = np.random.default_rng(r)
+        nv = 2 * int(rng.integers(4, 33))          # 8..64, even
+        xs, ys, qs = G.random_vortices(nv, seed=r)
+        e.set_ic_vortices(r, xs, ys, qs)
+    tune = e.autotune(3)
+    ext = torch.cuda.ExternalStream(e.stream)
+    e.rk4_async(warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(ext)
+    e.rk4_async(steps)
+    e1.record(ext)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    mode, graph = e.mode()
+    points = sum(N * N for N in sizes)
+    e.close()
+    names = ["all sizes per launch", "one launch per size", "one launch per realization"]
+    return {"workload": "384 realizations (128 x 128^2, 128 x 256^2, 128 x 512^2), 8-64 vortices, RK4 dt=1e-3",
+            "ensemble_steps_per_s": round(1e3 / ms, 2), "ms_per_step": round(ms, 4),
+            "realization_steps_per_s": round(len(sizes) * 1e3 / ms, 1),
+            "grid_point_steps_per_s": float("%.4g" % (points * 1e3 / ms)),
+            "schedule": {"launch": names[mode], "graph": graph},
+            "autotune_ms_per_step": {names[m]: {"graph": round(float(tune[m][0]), 4), "direct": round(float(tune[m][1]), 4)}
+                                     for m in range(3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +170,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pure", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the single-field path shards as "
@@ -180,6 +219,12 @@ def main():
     # (53 full-field transfers per step, 848 N^2 bytes)
     step_bytes_ref = 848.0 * N * N
 
+    ens = None
+    if not args.no_ensemble:
+        log("bench: ensemble")
+        ens = ensemble_bench(G, torch, max(10, args.steps // 4), 3)
+        log("bench: ensemble", ens)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -208,6 +253,7 @@ def main():
         "pure_gpe_steps_per_s": round(pure, 3) if pure else None,
         "effective_GBps_vs_53_transfer_step": round(step_bytes_ref * steps_per_s / 1e9, 1),
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

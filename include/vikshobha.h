@@ -126,6 +126,29 @@ int vk_pass_kinds(void);
 /* Compulsory HBM bytes one launch of kernel kind moves at this handle's N. */
 double vk_gpe_pass_bytes(vk_gpe* h, int kind);
 
+/* Ensemble of independent realizations (no reference counterpart: the
+ * reference runs one field per MPI job).  Members have N in {128, 256, 512,
+ * 1024}, no particles; each pass of an RK4 step is one launch over all
+ * members (mode 0), one per size (mode 1) or one per member (mode 2), and a
+ * step is replayed as a captured HIP graph (use_graph).  vk_ens_autotune
+ * times every schedule on the current state, restores the state and keeps
+ * the fastest (ms_out[6] = ms/step for modes 0..2 x {graph, direct}). */
+typedef struct vk_ens vk_ens;
+int vk_ens_create(vk_ens** e, int nreal, const vk_gpe_params* params);
+void vk_ens_destroy(vk_ens* e);
+int vk_ens_size(vk_ens* e);
+void* vk_ens_stream(vk_ens* e);
+int vk_ens_set_psi(vk_ens* e, int r, const double* psi_xy, int flags);
+int vk_ens_set_ic_vortices(vk_ens* e, int r, int nv, const double* xs, const double* ys,
+                           const int* qs, int nimg, int flags);
+int vk_ens_get_psi(vk_ens* e, int r, double* psi_xy, double* kpsi_t);
+int vk_ens_diagnostics(vk_ens* e, int r, double* scal, double* spec, double* pscal);
+int vk_ens_rk4_steps(vk_ens* e, int nsteps);
+int vk_ens_rk4_steps_async(vk_ens* e, int nsteps);
+int vk_ens_set_mode(vk_ens* e, int mode, int use_graph);
+int vk_ens_get_mode(vk_ens* e, int* mode, int* use_graph);
+int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out);
+
 const char* vk_last_error(void);
 const char* vk_version(void);
 

@@ -57,6 +57,19 @@ SIGNATURES = {
     "vk_pass_name": (ctypes.c_char_p, [ctypes.c_int]),
     "vk_pass_kinds": (ctypes.c_int, []),
     "vk_gpe_pass_bytes": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_ens_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Params)]),
+    "vk_ens_destroy": (None, [ctypes.c_void_p]),
+    "vk_ens_size": (ctypes.c_int, [ctypes.c_void_p]),
+    "vk_ens_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "vk_ens_set_psi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_int]),
+    "vk_ens_set_ic_vortices": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_int, ctypes.c_int]),
+    "vk_ens_get_psi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
+    "vk_ens_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]),
+    "vk_ens_rk4_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_ens_rk4_steps_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_ens_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "vk_ens_get_mode": (ctypes.c_int, [ctypes.c_void_p, ip, ip]),
+    "vk_ens_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
     "vk_last_error": (ctypes.c_char_p, []),
     "vk_version": (ctypes.c_char_p, []),
 }

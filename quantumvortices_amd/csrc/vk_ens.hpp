@@ -1,0 +1,71 @@
+// Batched ensemble of independent realizations of unequal size.  One launch
+// per pass serves every realization: each workgroup finds its realization
+// through a block map and runs the size's pass body.  Sizes 128..1024 all
+// use 256-thread workgroups, so one kernel holds every size.
+#pragma once
+#include "vk_kernels.hpp"
+
+namespace vk {
+
+struct EnsReal {
+    int N;
+    Geo g;
+    cplx *K0, *K1, *K2, *Hpsi, *Hrho, *HN;
+    const cplx* tw;
+    double alpha, omega, gsc, sc, dt;
+    cplx pref;
+};
+
+#define VK_ENS_SWITCH(N, BODY)                                           \
+    switch (N) {                                                          \
+    case 128: { using C = Cfg<128>; BODY; break; }                        \
+    case 256: { using C = Cfg<256>; BODY; break; }                        \
+    case 512: { using C = Cfg<512>; BODY; break; }                        \
+    case 1024: { using C = Cfg<1024>; BODY; break; }                      \
+    default: break;                                                       \
+    }
+
+#define VK_ENS_SETUP                                                      \
+    const int r = map[blockIdx.x];                                        \
+    const EnsReal& e = R[r];                                              \
+    const int lb = blockIdx.x - off[2 * r], nb = off[2 * r + 1] - off[2 * r];   // [start, end) of r
+
+__global__ void __launch_bounds__(256) k_ens_p1(const EnsReal* __restrict__ R, const int* __restrict__ map,
+                                                const int* __restrict__ off)
+{
+    VK_ENS_SETUP
+    VK_ENS_SWITCH(e.N, (p1_body<C>(lb, nb, e.Hpsi, e.Hrho, e.tw, e.g, 1, e.sc)))
+}
+
+__global__ void __launch_bounds__(256) k_ens_p2(const EnsReal* __restrict__ R, const int* __restrict__ map,
+                                                const int* __restrict__ off)
+{
+    VK_ENS_SETUP
+    VK_ENS_SWITCH(e.N, (p2_body<C>(lb, nb, e.Hrho, e.Hrho, (const cplx*)e.K1, e.K1, e.tw, e.g, 0)))
+}
+
+__global__ void __launch_bounds__(256) k_ens_p3(const EnsReal* __restrict__ R, const int* __restrict__ map,
+                                                const int* __restrict__ off)
+{
+    VK_ENS_SETUP
+    const P3Part pa{};
+    VK_ENS_SWITCH(e.N, (p3_body<C, 0>(lb, nb, e.Hrho, e.Hrho, e.Hpsi, e.HN, e.tw, e.g, 1, 1, e.gsc, e.sc, e.sc,
+                                      1.0, pa)))
+}
+
+__global__ void __launch_bounds__(256) k_ens_p4(const EnsReal* __restrict__ R, const int* __restrict__ map,
+                                                const int* __restrict__ off, int stage)
+{
+    VK_ENS_SETUP
+    RKArgs a{};
+    a.pref = e.pref;
+    a.alpha = e.alpha;
+    a.omega = e.omega;
+    a.stage = stage;
+    const double oney6 = 1.0 / 6.0;
+    a.wacc = (stage == 1 || stage == 4) ? e.dt * oney6 : e.dt * oney6 * 2.0;
+    a.cstep = stage == 3 ? e.dt : e.dt * 0.5;
+    VK_ENS_SWITCH(e.N, (p4_body<C>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a)))
+}
+
+}  // namespace vk

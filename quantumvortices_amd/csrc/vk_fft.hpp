@@ -256,7 +256,12 @@ template <int N, int S, int ST, bool SPLIT> struct Stage<N, S, ST, N, SPLIT> {
 template <int N, int S, bool SPLIT>
 __device__ __forceinline__ void fft_fwd(cplx (&v)[S], double* lds, int t, const cplx* __restrict__ tw)
 {
-    Stage<N, S, 0, 1, SPLIT>::run(v, lds, SPLIT ? lds : lds + Lds<N>::words, t, tw);
+    // Register-only compiler barrier on the (uniform) table pointer: each
+    // transform reloads its twiddles instead of keeping the previous
+    // transform's values live across the pointwise work in between.
+    const cplx* twb = tw;
+    asm volatile("" : "+s"(twb));
+    Stage<N, S, 0, 1, SPLIT>::run(v, lds, SPLIT ? lds : lds + Lds<N>::words, t, twb);
 }
 
 }  // namespace vk

@@ -8,12 +8,15 @@
 //   P4 column: N-hat = FFT_y HN ; RHS, RK4 accumulation ; Hpsi = IFFT_y kpsi
 // kpsi is dealiased, so only |kx| <= N/3 columns are stored and moved;
 // rho, rho_d and V are real, so only their kx >= 0 halves are moved.
-#include "vk_kernels.hpp"
+#include "vk_ens.hpp"
 #include "vikshobha.h"
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
+#include <array>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <set>
 #include <string>
@@ -85,6 +88,13 @@ struct vk_gpe {
     int *modes = nullptr, *offs = nullptr;
     double *dspec = nullptr, *dpart = nullptr;
     int diag_ready = 0;
+    // one-member ensemble table: N <= 1024 runs the same kernels as a
+    // batched ensemble member, so both give bit-identical results
+    vk::EnsReal* e1R = nullptr;
+    int* e1map[3] = {};
+    int* e1off[3] = {};
+    int e1nblk[3] = {};
+    size_t e1lds[4] = {};
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -372,8 +382,12 @@ void join_particles(vk_gpe* h)
 
 // ---- RK4 / ARGLE -----------------------------------------------------------------
 
+int ens1_step(vk_gpe* h);
+int make_single_table(vk_gpe* h);
+
 int rk4_step(vk_gpe* h)
 {
+    if (h->e1R && !(h->have_part && h->p.np > 0)) return ens1_step(h);
     return dispatch(h->N, [&](auto n) {
         using C = CfgN<decltype(n)::value>;
         const double sc = scale_of(h);
@@ -671,6 +685,10 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         h->KV = h->K1;
         h->Krho = h->K1;
     }
+    if (p->np == 0) {
+        int rc = make_single_table(h);
+        if (rc) return cleanup(rc);
+    }
     if (hipDeviceSynchronize() != hipSuccess) return cleanup(fail(VK_ERR_HIP, "create: device sync"));
     *out = h;
     return 0;
@@ -696,6 +714,8 @@ void vk_gpe_destroy(vk_gpe* h)
     if (h->dpart) hipFree(h->dpart);
     if (h->part) hipFree(h->part);
     if (h->dsum) hipFree(h->dsum);
+    if (h->e1R) hipFree(h->e1R);
+    for (int k = 0; k < 3; ++k) { if (h->e1map[k]) hipFree(h->e1map[k]); if (h->e1off[k]) hipFree(h->e1off[k]); }
     for (auto e : h->ev) hipEventDestroy(e);
     if (h->pstream) { hipStreamSynchronize(h->pstream); hipStreamDestroy(h->pstream); }
     if (h->ev_p3) hipEventDestroy(h->ev_p3);
@@ -952,6 +972,425 @@ double vk_gpe_pass_bytes(vk_gpe* h, int kind)
     case PK_P4: return F * (fr + fr) + D * (3 + 5 + 5 + 4) / 4.0;
     default: return 0.0;
     }
+}
+
+}  // extern "C"
+
+// =============================================================================
+// Ensemble of independent realizations (batched launches, graph replay)
+// =============================================================================
+
+struct vk_ens {
+    std::vector<vk_gpe*> h;
+    hipStream_t stream = nullptr;
+    vk::EnsReal* dR = nullptr;
+    // block maps: 0 row passes (P1, P3), 1 P2, 2 P4; per-size maps for mode 1
+    int* dmap[3] = {};
+    int* doff[3] = {};
+    int nblk[3] = {};
+    std::vector<int> sizes;                       // distinct N
+    std::map<int, std::vector<int>> size_reals;   // N -> realization ids
+    std::map<int, std::array<int*, 6>> smaps;     // N -> {map0, off0, map1, off1, map2, off2}
+    std::map<int, std::array<int, 3>> snblk;
+    size_t lds[4] = {};                           // max dynamic LDS per pass
+    int mode = 0;       // 0: one launch per pass for all sizes; 1: one per size; 2: per realization
+    int use_graph = 1;
+    hipGraphExec_t gexec[3] = {};
+    int synced = 0;
+};
+
+namespace {
+
+template <class C> void ens_lds(size_t* l)
+{
+    l[0] = std::max(l[0], (size_t)C::TR * C::ROWRS * sizeof(double));
+    l[1] = std::max(l[1], (size_t)C::TC * C::COLRS * sizeof(double));
+    l[2] = std::max(l[2], (size_t)C::TR * C::ROWRS * sizeof(double) + (size_t)C::S * C::ROWT * sizeof(double) + 16);
+    l[3] = std::max(l[3], (size_t)C::TC * C::COLRS * sizeof(double));
+}
+
+// Block maps for the realizations in ids: for pass kinds 0 (row passes),
+// 1 (P2) and 2 (P4), map[b] = realization of block b (runs contiguous, in
+// ids order) and off[2r], off[2r+1] = [start, end) of r's run.
+int build_maps(const std::vector<vk_gpe*>& hs, const std::vector<int>& ids, int** dmap, int** doff, int* nblk)
+{
+    for (int k = 0; k < 3; ++k) {
+        std::vector<int> map;
+        std::vector<int> se(2 * hs.size(), 0);
+        for (int r : ids) {
+            const vk_gpe* h = hs[r];
+            int TR = 0;
+            dispatch(h->N, [&](auto n) { TR = CfgN<decltype(n)::value>::TR; return 0; });
+            const int nb = k == 0 ? h->N / TR : (k == 1 ? h->geo.Pp : h->geo.Pp + h->geo.Pn);
+            se[2 * r] = (int)map.size();
+            for (int b = 0; b < nb; ++b) map.push_back(r);
+            se[2 * r + 1] = (int)map.size();
+        }
+        nblk[k] = (int)map.size();
+        HIPCHK(hipMalloc(&dmap[k], sizeof(int) * std::max<size_t>(1, map.size())));
+        HIPCHK(hipMalloc(&doff[k], sizeof(int) * se.size()));
+        HIPCHK(hipMemcpy(dmap[k], map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(doff[k], se.data(), sizeof(int) * se.size(), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// set the LDS limit of the single-field pass kernels of size C before any
+// graph capture
+template <class C> void preconfig_passes()
+{
+    set_lds(vk::k_p1<C>, (size_t)C::TR * C::ROWRS * sizeof(double));
+    set_lds(vk::k_p2<C>, (size_t)C::TC * C::COLRS * sizeof(double));
+    set_lds(vk::k_p3<C, 0>, (size_t)C::TR * C::ROWRS * sizeof(double) + (size_t)C::S * C::ROWT * sizeof(double) + 16);
+    set_lds(vk::k_p4<C>, (size_t)C::TC * C::COLRS * sizeof(double));
+}
+
+vk::EnsReal ens_entry(const vk_gpe* h)
+{
+    vk::EnsReal x{};
+    x.N = h->N;
+    x.g = h->geo;
+    x.K0 = h->K0; x.K1 = h->K1; x.K2 = h->K2; x.Hpsi = h->Hpsi; x.Hrho = h->Hrho; x.HN = h->HN;
+    x.tw = h->tw;
+    x.alpha = h->p.alpha;
+    x.omega = h->p.omega;
+    x.sc = scale_of(h);
+    x.gsc = h->p.g * x.sc;
+    x.dt = h->p.dt;
+    x.pref = make_double2(-h->p.disp, -1.0);
+    return x;
+}
+
+int make_single_table(vk_gpe* h)
+{
+    if (h->N < 128 || h->N > 1024) return 0;
+    const vk::EnsReal x = ens_entry(h);
+    HIPCHK(hipMalloc(&h->e1R, sizeof(vk::EnsReal)));
+    HIPCHK(hipMemcpy(h->e1R, &x, sizeof(x), hipMemcpyHostToDevice));
+    std::vector<vk_gpe*> hs{h};
+    int rc = build_maps(hs, std::vector<int>{0}, h->e1map, h->e1off, h->e1nblk);
+    if (rc) return rc;
+    dispatch(h->N, [&](auto n) { ens_lds<CfgN<decltype(n)::value>>(h->e1lds); return 0; });
+    set_lds(vk::k_ens_p1, h->e1lds[0]);
+    set_lds(vk::k_ens_p2, h->e1lds[1]);
+    set_lds(vk::k_ens_p3, h->e1lds[2]);
+    set_lds(vk::k_ens_p4, h->e1lds[3]);
+    return 0;
+}
+
+// launch the four passes of every stage over the given block maps
+void ens_passes(hipStream_t st, const vk::EnsReal* R, int* const* m, int* const* o, const int* nb, const size_t* lds,
+                vk_gpe* th)
+{
+    for (int stage = 1; stage <= 4; ++stage) {
+        if (th) ev_begin(th);
+        hipLaunchKernelGGL(vk::k_ens_p1, dim3(nb[0]), dim3(256), lds[0], st, R, (const int*)m[0], (const int*)o[0]);
+        if (th) ev_end(th, PK_P1), ev_begin(th);
+        hipLaunchKernelGGL(vk::k_ens_p2, dim3(nb[1]), dim3(256), lds[1], st, R, (const int*)m[1], (const int*)o[1]);
+        if (th) ev_end(th, PK_P2), ev_begin(th);
+        hipLaunchKernelGGL(vk::k_ens_p3, dim3(nb[0]), dim3(256), lds[2], st, R, (const int*)m[0], (const int*)o[0]);
+        if (th) ev_end(th, PK_P3), ev_begin(th);
+        hipLaunchKernelGGL(vk::k_ens_p4, dim3(nb[2]), dim3(256), lds[3], st, R, (const int*)m[2], (const int*)o[2], stage);
+        if (th) ev_end(th, PK_P4);
+    }
+}
+
+int ens1_step(vk_gpe* h)
+{
+    ens_passes(h->stream, h->e1R, h->e1map, h->e1off, h->e1nblk, h->e1lds, h);
+    h->kv_valid = 0;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vk_ens_create(vk_ens** out, int nreal, const vk_gpe_params* params)
+{
+    if (!out || nreal <= 0 || !params) return fail(VK_ERR_ARG, "ens_create: bad arguments");
+    *out = nullptr;
+    for (int r = 0; r < nreal; ++r) {
+        const int N = params[r].N;
+        if (N != 128 && N != 256 && N != 512 && N != 1024)
+            return fail(VK_ERR_UNSUPPORTED, "ensemble members must have N in {128, 256, 512, 1024}");
+        if (params[r].np != 0) return fail(VK_ERR_UNSUPPORTED, "ensemble members carry no particles");
+    }
+    vk_ens* e = new vk_ens();
+    auto cleanup = [&](int rc) { vk_ens_destroy(e); return rc; };
+    for (int r = 0; r < nreal; ++r) {
+        vk_gpe* h = nullptr;
+        int rc = vk_gpe_create(&h, &params[r]);
+        if (rc) return cleanup(rc);
+        e->h.push_back(h);
+        e->size_reals[h->N].push_back(r);
+    }
+    for (auto& kv : e->size_reals) e->sizes.push_back(kv.first);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(VK_ERR_HIP, "ens stream"));
+    std::vector<vk::EnsReal> R(nreal);
+    for (int r = 0; r < nreal; ++r) {
+        vk_gpe* h = e->h[r];
+        R[r] = ens_entry(h);
+        dispatch(h->N, [&](auto n) { ens_lds<CfgN<decltype(n)::value>>(e->lds); return 0; });
+    }
+    if (hipMalloc(&e->dR, sizeof(vk::EnsReal) * nreal) != hipSuccess ||
+        hipMemcpy(e->dR, R.data(), sizeof(vk::EnsReal) * nreal, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(VK_ERR_HIP, "ens table"));
+    std::vector<int> all(nreal);
+    for (int r = 0; r < nreal; ++r) all[r] = r;
+    int rc = build_maps(e->h, all, e->dmap, e->doff, e->nblk);
+    if (rc) return cleanup(rc);
+    for (int N : e->sizes) {
+        std::array<int*, 6> m{};
+        int* dm[3];
+        int* dof[3];
+        std::array<int, 3> nb{};
+        if ((rc = build_maps(e->h, e->size_reals[N], dm, dof, nb.data()))) return cleanup(rc);
+        for (int k = 0; k < 3; ++k) { m[2 * k] = dm[k]; m[2 * k + 1] = dof[k]; }
+        e->smaps[N] = m;
+        e->snblk[N] = nb;
+    }
+    for (int N : e->sizes) dispatch(N, [&](auto n) { preconfig_passes<CfgN<decltype(n)::value>>(); return 0; });
+    set_lds(vk::k_ens_p1, e->lds[0]);
+    set_lds(vk::k_ens_p2, e->lds[1]);
+    set_lds(vk::k_ens_p3, e->lds[2]);
+    set_lds(vk::k_ens_p4, e->lds[3]);
+    *out = e;
+    return 0;
+}
+
+void vk_ens_destroy(vk_ens* e)
+{
+    if (!e) return;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    for (auto g : e->gexec) if (g) hipGraphExecDestroy(g);
+    for (int k = 0; k < 3; ++k) { if (e->dmap[k]) hipFree(e->dmap[k]); if (e->doff[k]) hipFree(e->doff[k]); }
+    for (auto& kv : e->smaps) for (auto p : kv.second) if (p) hipFree(p);
+    if (e->dR) hipFree(e->dR);
+    for (auto h : e->h) vk_gpe_destroy(h);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+}  // extern "C"
+
+namespace {
+
+// one RK4 step of every realization on the ensemble stream
+int ens_step_launch(vk_ens* e)
+{
+    if (e->mode == 2) {
+        // per realization: the single-field kernels, one launch per pass each
+        for (auto h : e->h) {
+            hipStream_t keep = h->stream;
+            h->stream = e->stream;
+            int rc = rk4_step(h);
+            h->stream = keep;
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    struct Grp { int* m[3]; int* o[3]; int nb[3]; };
+    std::vector<Grp> gs;
+    if (e->mode == 0) {
+        gs.push_back({{e->dmap[0], e->dmap[1], e->dmap[2]}, {e->doff[0], e->doff[1], e->doff[2]},
+                      {e->nblk[0], e->nblk[1], e->nblk[2]}});
+    } else {
+        for (int N : e->sizes) {
+            auto& m = e->smaps[N];
+            auto& nb = e->snblk[N];
+            gs.push_back({{m[0], m[2], m[4]}, {m[1], m[3], m[5]}, {nb[0], nb[1], nb[2]}});
+        }
+    }
+    if (gs.size() == 1) {
+        ens_passes(e->stream, e->dR, gs[0].m, gs[0].o, gs[0].nb, e->lds, nullptr);
+        return 0;
+    }
+    // one launch per size, sizes interleaved pass by pass
+    for (int stage = 1; stage <= 4; ++stage) {
+        for (auto& g : gs)
+            hipLaunchKernelGGL(vk::k_ens_p1, dim3(g.nb[0]), dim3(256), e->lds[0], e->stream,
+                               (const vk::EnsReal*)e->dR, (const int*)g.m[0], (const int*)g.o[0]);
+        for (auto& g : gs)
+            hipLaunchKernelGGL(vk::k_ens_p2, dim3(g.nb[1]), dim3(256), e->lds[1], e->stream,
+                               (const vk::EnsReal*)e->dR, (const int*)g.m[1], (const int*)g.o[1]);
+        for (auto& g : gs)
+            hipLaunchKernelGGL(vk::k_ens_p3, dim3(g.nb[0]), dim3(256), e->lds[2], e->stream,
+                               (const vk::EnsReal*)e->dR, (const int*)g.m[0], (const int*)g.o[0]);
+        for (auto& g : gs)
+            hipLaunchKernelGGL(vk::k_ens_p4, dim3(g.nb[2]), dim3(256), e->lds[3], e->stream,
+                               (const vk::EnsReal*)e->dR, (const int*)g.m[2], (const int*)g.o[2], stage);
+    }
+    return 0;
+}
+
+int ens_prepare(vk_ens* e)
+{
+    if (e->synced) return 0;
+    for (auto h : e->h) {
+        int rc = ensure_rk4(h);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    e->synced = 1;
+    return 0;
+}
+
+int ens_steps_async(vk_ens* e, int nsteps)
+{
+    int rc = ens_prepare(e);
+    if (rc) return rc;
+    if (e->use_graph) {
+        hipGraphExec_t& ge = e->gexec[e->mode];
+        if (!ge) {
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            rc = ens_step_launch(e);
+            hipError_t ce = hipStreamEndCapture(e->stream, &g);
+            if (rc) return rc;
+            HIPCHK(ce);
+            HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(g));
+        }
+        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(ge, e->stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i)
+            if ((rc = ens_step_launch(e))) return rc;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vk_ens_size(vk_ens* e) { return e ? (int)e->h.size() : 0; }
+void* vk_ens_stream(vk_ens* e) { return e ? (void*)e->stream : nullptr; }
+
+static int ens_member(vk_ens* e, int r, vk_gpe** h)
+{
+    if (!e || r < 0 || r >= (int)e->h.size()) return fail(VK_ERR_ARG, "ensemble: bad realization index");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    *h = e->h[r];
+    return 0;
+}
+
+int vk_ens_set_psi(vk_ens* e, int r, const double* psi, int flags)
+{
+    vk_gpe* h;
+    int rc = ens_member(e, r, &h);
+    if (rc) return rc;
+    e->synced = 0;
+    return vk_gpe_set_psi(h, psi, flags);
+}
+
+int vk_ens_set_ic_vortices(vk_ens* e, int r, int nv, const double* xs, const double* ys, const int* qs, int nimg,
+                           int flags)
+{
+    vk_gpe* h;
+    int rc = ens_member(e, r, &h);
+    if (rc) return rc;
+    e->synced = 0;
+    return vk_gpe_set_ic_vortices(h, nv, xs, ys, qs, nimg, flags);
+}
+
+int vk_ens_get_psi(vk_ens* e, int r, double* psi, double* kpsi)
+{
+    vk_gpe* h;
+    int rc = ens_member(e, r, &h);
+    if (rc) return rc;
+    return vk_gpe_get_psi(h, psi, kpsi);
+}
+
+int vk_ens_diagnostics(vk_ens* e, int r, double* scal, double* spec, double* pscal)
+{
+    vk_gpe* h;
+    int rc = ens_member(e, r, &h);
+    if (rc) return rc;
+    return vk_gpe_diagnostics(h, scal, spec, pscal);
+}
+
+int vk_ens_rk4_steps_async(vk_ens* e, int nsteps)
+{
+    if (!e || nsteps < 0) return fail(VK_ERR_ARG, "ens_rk4_steps: bad arguments");
+    return ens_steps_async(e, nsteps);
+}
+
+int vk_ens_rk4_steps(vk_ens* e, int nsteps)
+{
+    int rc = vk_ens_rk4_steps_async(e, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vk_ens_set_mode(vk_ens* e, int mode, int use_graph)
+{
+    if (!e || mode < 0 || mode > 2) return fail(VK_ERR_ARG, "ens_set_mode: bad arguments");
+    e->mode = mode;
+    e->use_graph = use_graph ? 1 : 0;
+    return 0;
+}
+
+int vk_ens_get_mode(vk_ens* e, int* mode, int* use_graph)
+{
+    if (!e) return fail(VK_ERR_ARG, "ens_get_mode: null");
+    if (mode) *mode = e->mode;
+    if (use_graph) *use_graph = e->use_graph;
+    return 0;
+}
+
+// Autotuner: times every (launch mode, graph) schedule over nsteps steps from
+// the current state, restores the state, keeps the fastest.  ms_out[6]:
+// ms per step for modes 0,1,2 x {graph, direct}.
+int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
+{
+    if (!e || nsteps <= 0) return fail(VK_ERR_ARG, "ens_autotune: bad arguments");
+    int rc = ens_prepare(e);
+    if (rc) return rc;
+    // save the state (kpsi and its y-inverse); K1/K2 are rewritten by stage 1
+    std::vector<cplx*> save;
+    for (auto h : e->h) {
+        cplx* b = nullptr;
+        HIPCHK(hipMalloc(&b, 2 * f2(h) * sizeof(cplx)));
+        HIPCHK(hipMemcpyAsync(b, h->K0, f2(h) * sizeof(cplx), hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(b + f2(h), h->Hpsi, f2(h) * sizeof(cplx), hipMemcpyDeviceToDevice, e->stream));
+        save.push_back(b);
+    }
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    double best = 1e300;
+    int bm = 0, bg = 1;
+    for (int m = 0; m < 3; ++m)
+        for (int gph = 1; gph >= 0; --gph) {
+            e->mode = m;
+            e->use_graph = gph;
+            if ((rc = ens_steps_async(e, 1))) return rc;   // warm (and capture)
+            HIPCHK(hipEventRecord(a, e->stream));
+            if ((rc = ens_steps_async(e, nsteps))) return rc;
+            HIPCHK(hipEventRecord(b, e->stream));
+            HIPCHK(hipEventSynchronize(b));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, a, b));
+            const double per = ms / nsteps;
+            if (ms_out) ms_out[2 * m + (1 - gph)] = per;
+            if (per < best) { best = per; bm = m; bg = gph; }
+        }
+    e->mode = bm;
+    e->use_graph = bg;
+    for (size_t i = 0; i < e->h.size(); ++i) {
+        vk_gpe* h = e->h[i];
+        HIPCHK(hipMemcpyAsync(h->K0, save[i], f2(h) * sizeof(cplx), hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(h->Hpsi, save[i] + f2(h), f2(h) * sizeof(cplx), hipMemcpyDeviceToDevice, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (auto p : save) hipFree(p);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return 0;
 }
 
 }  // extern "C"

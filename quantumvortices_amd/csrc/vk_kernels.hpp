@@ -37,7 +37,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     const int q_ = u % (TR * TC);                                          \
     const int rr = q_ / TC, c = q_ % TC, pidx0 = u / (TR * TC);            \
     const int t = TC * pidx0 + c;                                          \
-    const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + rr;             \
+    const int y = xcd_remap(blk_, nblk_) * TR + rr;                       \
     double* lg = lds + rr * C::ROWRS;
 
 #define VK_COL_SETUP(C, PEXPR)                                             \
@@ -65,6 +65,7 @@ template <class C>
 __global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd(const cplx* __restrict__ X, cplx* __restrict__ H,
                                                      const cplx* __restrict__ tw, Geo g, int pstore, int op)
 {
+    const int blk_ = blockIdx.x, nblk_ = gridDim.x;
     VK_ROW_SETUP(C)
     cplx v[S];
     const cplx* xr = X + (size_t)y * N;
@@ -134,6 +135,7 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv(const cplx* __restri
                                                      const cplx* __restrict__ tw, Geo g, int pload, int op,
                                                      double scale)
 {
+    const int blk_ = blockIdx.x, nblk_ = gridDim.x;
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
@@ -180,7 +182,7 @@ __global__ void k_dealias(cplx* __restrict__ K, int N)
 // P1 (row): psi row from Hpsi (inverse along x, times 1/N^2), rho = |psi|^2,
 // forward along x, keep kx >= 0 retained panels (rho is real).
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+__device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
                                                 double scale)
 {
@@ -205,16 +207,24 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ 
     }
 }
 
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                double scale)
+{
+    p1_body<C>(blockIdx.x, gridDim.x, Hpsi, Hrho, tw, g, psi_pmode, scale);
+}
+
 // P2 (column, kx >= 0 retained panels): forward along y, dealias
 // (P_G), inverse along y, in place.  flags & 1: also the particle
 // potential column HV = IFFT_y(KV); flags & 2: store the dealiased
 // rho-hat row into K-space Krho (for the particle force).
 template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
+__device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
-    VK_COL_SETUP(C, blockIdx.x)
+    VK_COL_SETUP(C, blk_)
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = Hrho[hoff<C>(p, t + T * s, c)];
@@ -240,6 +250,14 @@ __global__ void __launch_bounds__(C::COLT) VK_OCC k_p2(cplx* __restrict__ Hrho, 
 #pragma unroll
         for (int s = 0; s < S; ++s) HV[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
     }
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
+                                                const cplx* __restrict__ KV, cplx* __restrict__ Krho,
+                                                const cplx* __restrict__ tw, Geo g, int flags)
+{
+    p2_body<C>(blockIdx.x, gridDim.x, Hrho, HV, KV, Krho, tw, g, flags);
 }
 
 // deterministic workgroup sum of NV values per thread; result valid in
@@ -324,7 +342,7 @@ __device__ __forceinline__ void pcoord(double a, double& X, double& dX)
 // 2 real-space potential; 6 real-space potential + force partial sums.
 // vsign: +1 RK4, -1 ARGLE.
 template <class C, int PM>
-__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+__device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
                                                 const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
                                                 int store_pmode, double gs, double vs, double scale,
@@ -373,7 +391,7 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
         double rho[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) rho[s] = v[s].x;
-        const int blk = xcd_remap(blockIdx.x, gridDim.x);
+        const int blk = xcd_remap(blk_, nblk_);
         const int yb = blk * TR;
         const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
         const double idx = 1.0 / pa.dx, Wg = pa.W * idx, invN = 1.0 / N;
@@ -443,6 +461,16 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
     }
 }
 
+template <class C, int PM>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+                                                const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
+                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                int store_pmode, double gs, double vs, double scale,
+                                                double vsign, P3Part pa)
+{
+    p3_body<C, PM>(blockIdx.x, gridDim.x, Hrhod, HV, Hpsi, HN, tw, g, psi_pmode, store_pmode, gs, vs, scale, vsign, pa);
+}
+
 struct RKArgs {
     cplx pref;      // (1 - i disp)/i
     double alpha, omega;
@@ -457,12 +485,12 @@ struct RKArgs {
 // inverse along y of the new kpsi -> Hpsi.
 // K0 = str1 (= kpsi at step start), K1 = str2, K2 = stage kpsi.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+__device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
                                                 cplx* __restrict__ K2, const cplx* __restrict__ tw,
                                                 Geo g, RKArgs a)
 {
-    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, 1))
+    VK_COL_SETUP(C, panel_of_block(blk_, C::NP, g, 1))
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
@@ -499,6 +527,15 @@ __global__ void __launch_bounds__(C::COLT) VK_OCC k_p4(const cplx* __restrict__ 
     for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
 }
 
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+                                                cplx* __restrict__ K0, cplx* __restrict__ K1,
+                                                cplx* __restrict__ K2, const cplx* __restrict__ tw,
+                                                Geo g, RKArgs a)
+{
+    p4_body<C>(blockIdx.x, gridDim.x, HN, Hpsi, K0, K1, K2, tw, g, a);
+}
+
 struct ArgleArgs {
     double alpha, omega, dt;
     int want_res;
@@ -509,11 +546,11 @@ struct ArgleArgs {
 // (src/gpe.f90:195-197, no dealiasing of the result); residual partial
 // sums; inverse along y of the new kpsi -> Hpsi.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+__device__ __forceinline__ void a4_body(int blk_, int nblk_, const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, const cplx* __restrict__ tw,
                                                 ArgleArgs a, double* __restrict__ part)
 {
-    VK_COL_SETUP(C, blockIdx.x)
+    VK_COL_SETUP(C, blk_)
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
@@ -546,10 +583,18 @@ __global__ void __launch_bounds__(C::COLT) VK_OCC k_a4(const cplx* __restrict__ 
     if (a.want_res) {
         block_sum<2>(r, lds);
         if (threadIdx.x == 0) {
-            part[2 * blockIdx.x] = r[0];
-            part[2 * blockIdx.x + 1] = r[1];
+            part[2 * blk_] = r[0];
+            part[2 * blk_ + 1] = r[1];
         }
     }
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+                                                cplx* __restrict__ K0, const cplx* __restrict__ tw,
+                                                ArgleArgs a, double* __restrict__ part)
+{
+    a4_body<C>(blockIdx.x, gridDim.x, HN, Hpsi, K0, tw, a, part);
 }
 
 // fixed-order reduction of nb partial records of nv doubles -> out[nv]

@@ -167,6 +167,84 @@ class GPE:
         return {names[i]: nat.lib().vk_gpe_pass_bytes(self.h, i) for i in range(k)}
 
 
+class Ensemble:
+    """Independent realizations of unequal size stepped together: every pass
+    of an RK4 step is one batched launch, and a step replays as a captured
+    HIP graph (no reference counterpart; the reference runs one field per
+    MPI job)."""
+
+    def __init__(self, params_list):
+        n = len(params_list)
+        arr = (nat.Params * n)(*params_list)
+        self.params = list(params_list)
+        h = nat.ctypes.c_void_p()
+        nat.check(nat.lib().vk_ens_create(nat.ctypes.byref(h), n, arr))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            nat.lib().vk_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __len__(self):
+        return len(self.params)
+
+    def set_psi(self, r, psi, sync=True):
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        nat.check(nat.lib().vk_ens_set_psi(self.h, int(r), nat.ptr(a), 1 if sync else 0))
+
+    def set_ic_vortices(self, r, xs, ys, qs, nimg=3, sync=True):
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        ys = np.ascontiguousarray(ys, dtype=np.float64)
+        qs = np.ascontiguousarray(qs, dtype=np.int32)
+        nat.check(nat.lib().vk_ens_set_ic_vortices(self.h, int(r), len(xs), nat.ptr(xs), nat.ptr(ys),
+                                                   qs.ctypes.data_as(nat.ip), int(nimg), 1 if sync else 0))
+
+    def psi(self, r):
+        N = self.params[r].N
+        out = np.empty((N, N), dtype=np.complex128)
+        nat.check(nat.lib().vk_ens_get_psi(self.h, int(r), nat.ptr(out), None))
+        return out
+
+    def kpsi(self, r):
+        N = self.params[r].N
+        out = np.empty((N, N), dtype=np.complex128)
+        nat.check(nat.lib().vk_ens_get_psi(self.h, int(r), None, nat.ptr(out)))
+        return out
+
+    def diagnostics(self, r):
+        ns = nshell(self.params[r].N)
+        scal, spec, pscal = np.zeros(8), np.zeros(6 * (ns + 1)), np.zeros(5)
+        nat.check(nat.lib().vk_ens_diagnostics(self.h, int(r), nat.ptr(scal), nat.ptr(spec), nat.ptr(pscal)))
+        return scal, spec.reshape(6, ns + 1)
+
+    def rk4(self, nsteps):
+        nat.check(nat.lib().vk_ens_rk4_steps(self.h, int(nsteps)))
+
+    def rk4_async(self, nsteps):
+        nat.check(nat.lib().vk_ens_rk4_steps_async(self.h, int(nsteps)))
+
+    @property
+    def stream(self):
+        return nat.lib().vk_ens_stream(self.h)
+
+    def set_mode(self, mode, graph=True):
+        nat.check(nat.lib().vk_ens_set_mode(self.h, int(mode), 1 if graph else 0))
+
+    def mode(self):
+        m, g = nat.ctypes.c_int(), nat.ctypes.c_int()
+        nat.check(nat.lib().vk_ens_get_mode(self.h, nat.ctypes.byref(m), nat.ctypes.byref(g)))
+        return m.value, bool(g.value)
+
+    def autotune(self, nsteps=3):
+        ms = np.zeros(6)
+        nat.check(nat.lib().vk_ens_autotune(self.h, int(nsteps), nat.ptr(ms)))
+        return ms.reshape(3, 2)
+
+
 def ic_vortices_host(N, xs, ys, qs, xi, nimg=3):
     """Padé-core vortex product on the CPU (small grids; input generation)."""
     xs = np.ascontiguousarray(xs, dtype=np.float64)
