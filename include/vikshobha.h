@@ -82,6 +82,10 @@ int vk_gpe_rk4_steps_async(vk_gpe* h, int nsteps);
  * NULL) receives ||kpsi_new-kpsi_old||_2/(dt*||kpsi_old||_2) of the last
  * step; the reference computes no residual. */
 int vk_gpe_argle_steps(vk_gpe* h, int nsteps, double* residual);
+/* ARGLE until the residual (as above, evaluated every check_every steps)
+ * is <= tol or max_steps are done; steps_out receives the steps taken. */
+int vk_gpe_argle_until(vk_gpe* h, double tol, int max_steps, int check_every,
+                       int* steps_out, double* residual);
 
 /* diagnostics_gpe energies and spectra (src/diagnostics.f90:54-426):
  * scal[8] = {egrad, echem, eintr, etot, norm, momx, momy, n0=spec(0,6)};

@@ -46,6 +46,7 @@ SIGNATURES = {
     "vk_gpe_rk4_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "vk_gpe_rk4_steps_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "vk_gpe_argle_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
+    "vk_gpe_argle_until": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, dp]),
     "vk_gpe_diagnostics": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
     "vk_gpe_particles_set": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
     "vk_gpe_particles_get": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),

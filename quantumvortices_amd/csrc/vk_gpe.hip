@@ -819,6 +819,30 @@ int vk_gpe_argle_steps(vk_gpe* h, int nsteps, double* residual)
     return 0;
 }
 
+int vk_gpe_argle_until(vk_gpe* h, double tol, int max_steps, int check_every, int* steps_out, double* residual)
+{
+    if (!h || max_steps < 0 || check_every <= 0 || tol <= 0.0) return fail(VK_ERR_ARG, "argle_until: bad arguments");
+    int rc = ensure_full(h);
+    if (rc) return rc;
+    int done = 0;
+    double res = INFINITY;
+    while (done < max_steps) {
+        const int n = std::min(check_every, max_steps - done);
+        for (int i = 0; i < n; ++i)
+            if ((rc = argle_step(h, i == n - 1))) return rc;
+        done += n;
+        double r[2];
+        HIPCHK(hipMemcpyAsync(r, h->dsum, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        res = std::sqrt(r[0]) / (h->p.dt * std::sqrt(r[1]));
+        if (!(res > tol)) break;   // converged (or NaN: stop and report)
+    }
+    HIPCHK(hipGetLastError());
+    if (steps_out) *steps_out = done;
+    if (residual) *residual = res;
+    return 0;
+}
+
 int vk_gpe_diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal)
 {
     if (!h) return fail(VK_ERR_ARG, "diagnostics: null handle");

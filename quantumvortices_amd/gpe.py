@@ -105,6 +105,14 @@ class GPE:
         nat.check(nat.lib().vk_gpe_argle_steps(self.h, int(nsteps), nat.ptr(r)))
         return float(r[0])
 
+    def argle_until(self, tol=1e-10, max_steps=10 ** 6, check_every=500):
+        """ARGLE until residual <= tol; returns (steps, residual)."""
+        n = nat.ctypes.c_int()
+        r = np.zeros(1)
+        nat.check(nat.lib().vk_gpe_argle_until(self.h, float(tol), int(max_steps), int(check_every),
+                                               nat.ctypes.byref(n), nat.ptr(r)))
+        return n.value, float(r[0])
+
     def diagnostics_gpe(self, spoint=0, jtime=0, time=0.0):
         """diagnostics_gpe (src/diagnostics.f90:4-571): returns a dict with
         egrad, echem, eintr, etot, norm, momx, momy, n0, spec (6, nshell+1)
