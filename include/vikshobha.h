@@ -68,7 +68,10 @@ void vk_gpe_destroy(vk_gpe* h);    /* clear_fft + free_state_variables */
 
 /* Field upload/download.  set_psi: kpsi = FFT(psi) (src/initial_config.f90:759)
  * and, with VK_SYNC_DEALIAS, the dealias + psi=IFFT(kpsi) sync of
- * ic_otoc_gpe.  get_psi: either pointer may be NULL. */
+ * ic_otoc_gpe.  get_psi: either pointer may be NULL.  The arrays may be host
+ * or device pointers; device data must be complete when the call is made
+ * (the copies run on the handle's stream, which does not wait for other
+ * streams), and each call returns after its copies have finished. */
 int vk_gpe_set_psi(vk_gpe* h, const double* psi_xy, int flags);
 int vk_gpe_set_kpsi(vk_gpe* h, const double* kpsi_t);
 int vk_gpe_get_psi(vk_gpe* h, double* psi_xy, double* kpsi_t);
@@ -101,6 +104,15 @@ int vk_gpe_diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal);
 int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel,
                          const double* fext);
 int vk_gpe_particles_get(vk_gpe* h, double* pos, double* vel, double* frc);
+
+/* Static obstacles for ARGLE (which never moves particles, src/gpe.f90:45-50):
+ * replaces the particle count (at most the np the handle was created with),
+ * positions pos[2*np], V0 and d, recomputes the potential and clears
+ * velocities.  np = 0 removes the potential. */
+int vk_gpe_set_obstacles(vk_gpe* h, int np, const double* pos, double V0, double d);
+
+/* Replaces the time step (dt_fix) of a live handle. */
+int vk_gpe_set_dt(vk_gpe* h, double dt);
 
 /* Conformance hook mirroring fft(plan_name, direction)
  * (src/mod_fft.f90:471-542): dir = -1: in = x-space (Nx,Ny) -> out = k-space
@@ -152,6 +164,54 @@ int vk_ens_rk4_steps_async(vk_ens* e, int nsteps);
 int vk_ens_set_mode(vk_ens* e, int mode, int use_graph);
 int vk_ens_get_mode(vk_ens* e, int* mode, int* use_graph);
 int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out);
+
+/* Fourier neural operator for the ARGLE warm start (no reference counterpart:
+ * the reference starts ARGLE from the analytic ansatz of
+ * src/initial_config.f90).  BF16 weights, FP32 activations and accumulation;
+ * forward, backward and Adam run on the handle's stream.  Arrays are planar
+ * FP32 [nb][channels][N][N] with x fastest and may be host or device
+ * pointers (device data complete at the call, as for vk_gpe_set_psi).
+ * Supported (width, modes): (32,16), (16,8), (8,4).
+ *  vk_fno_layout: offsets of Wp, bp, {R, W, b} per layer, Q1, bq1, Q2, bq2
+ *    and the total, in floats (returns the number written, 5 + 3*layers+2).
+ *  vk_fno_set_params: FP32 values, rounded to BF16; resets Adam.
+ *  vk_fno_get_params: the BF16 weights the network computes with.
+ *  vk_fno_train_step: forward, mean-squared error against tgt [nb][2][N][N],
+ *    backward (gradient readable by vk_fno_get_grad) and, for lr > 0, one
+ *    Adam step (beta 0.9/0.999, eps 1e-8).
+ *  vk_fno_predict_psi: one sample -> psi as double[2*N*N] (x, y), x fastest,
+ *    ready for vk_gpe_set_psi; x = NULL uses the planes vk_fno_features built.
+ *    flags VK_FNO_REAL keeps Re psi only (the ground state of a real
+ *    potential is real; an imaginary error would excite phase modes that
+ *    ARGLE damps only at rate alpha k^2); VK_FNO_ADD_TF adds input plane 2
+ *    (the Thomas-Fermi profile) to Re psi, for a network trained on the
+ *    correction psi - TF.
+ *  vk_fno_features: input planes of one sample on the GPU: plane 0 =
+ *    sum_v q_v exp(-r^2/(2 sigma^2)) over the vortices, plane 1 =
+ *    (V0/vnorm) sum_p exp(-r^2/(2 d^2)) over the obstacles, with r from the
+ *    particle potential's periodic coordinate (src/gpe_particle.f90:31-45);
+ *    with cin >= 3, plane 2 = sqrt(max(0, (omega - V)/g)), V = vnorm plane 1;
+ *    further planes are zero.  Kept in the handle and copied to x_out unless
+ *    it is NULL.
+ *  vk_fno_tf_guess: psi = sqrt(max(0, (omega - V)/g)) from plane 1. */
+typedef struct vk_fno vk_fno;
+int vk_fno_create(vk_fno** f, int N, int cin, int width, int modes, int layers, int proj, int batch);
+void vk_fno_destroy(vk_fno* f);
+long vk_fno_nparams(const vk_fno* f);
+int vk_fno_layout(const vk_fno* f, long* offsets);
+void* vk_fno_stream(const vk_fno* f);
+int vk_fno_set_params(vk_fno* f, const float* params);
+int vk_fno_get_params(vk_fno* f, float* params);
+int vk_fno_get_grad(vk_fno* f, float* grad);
+int vk_fno_forward(vk_fno* f, const float* x, float* y, int nb);
+#define VK_FNO_REAL 1
+#define VK_FNO_ADD_TF 2
+int vk_fno_predict_psi(vk_fno* f, const float* x, double* psi_xy, int flags);
+int vk_fno_features(vk_fno* f, int nv, const double* xv, const double* yv, const int* qv, double sigma,
+                    int nobs, const double* pos, double V0, double d, double vnorm, double omega,
+                    double g, float* x_out);
+int vk_fno_tf_guess(vk_fno* f, double omega, double g, double vnorm, double* psi_xy);
+int vk_fno_train_step(vk_fno* f, const float* x, const float* tgt, int nb, float lr, double* loss);
 
 const char* vk_last_error(void);
 const char* vk_version(void);

@@ -14,6 +14,8 @@ _lib = None
 
 dp = ctypes.POINTER(ctypes.c_double)
 ip = ctypes.POINTER(ctypes.c_int)
+fp = ctypes.POINTER(ctypes.c_float)
+lp = ctypes.POINTER(ctypes.c_long)
 
 
 class Params(ctypes.Structure):
@@ -50,6 +52,8 @@ SIGNATURES = {
     "vk_gpe_diagnostics": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
     "vk_gpe_particles_set": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
     "vk_gpe_particles_get": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
+    "vk_gpe_set_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "vk_gpe_set_obstacles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]),
     "vk_fft2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
     "vk_ic_vortices_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_double, ctypes.c_int, dp]),
     "vk_gpe_set_ic_vortices": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, ip, ctypes.c_int, ctypes.c_int]),
@@ -71,6 +75,21 @@ SIGNATURES = {
     "vk_ens_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "vk_ens_get_mode": (ctypes.c_int, [ctypes.c_void_p, ip, ip]),
     "vk_ens_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
+    "vk_fno_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)] + [ctypes.c_int] * 7),
+    "vk_fno_destroy": (None, [ctypes.c_void_p]),
+    "vk_fno_nparams": (ctypes.c_long, [ctypes.c_void_p]),
+    "vk_fno_layout": (ctypes.c_int, [ctypes.c_void_p, lp]),
+    "vk_fno_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "vk_fno_set_params": (ctypes.c_int, [ctypes.c_void_p, fp]),
+    "vk_fno_get_params": (ctypes.c_int, [ctypes.c_void_p, fp]),
+    "vk_fno_get_grad": (ctypes.c_int, [ctypes.c_void_p, fp]),
+    "vk_fno_forward": (ctypes.c_int, [ctypes.c_void_p, fp, fp, ctypes.c_int]),
+    "vk_fno_predict_psi": (ctypes.c_int, [ctypes.c_void_p, fp, dp, ctypes.c_int]),
+    "vk_fno_features": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, ip, ctypes.c_double, ctypes.c_int, dp,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, fp]),
+    "vk_fno_tf_guess": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]),
+    "vk_fno_train_step": (ctypes.c_int, [ctypes.c_void_p, fp, fp, ctypes.c_int, ctypes.c_float, dp]),
     "vk_last_error": (ctypes.c_char_p, []),
     "vk_version": (ctypes.c_char_p, []),
 }
@@ -103,4 +122,18 @@ def check(rc):
 
 
 def ptr(a):
+    return a.ctypes.data_as(dp)
+
+
+def fptr(a):
+    """float* of a float32 numpy array or of a device tensor (data_ptr())."""
+    if hasattr(a, "data_ptr"):
+        return ctypes.cast(ctypes.c_void_p(a.data_ptr()), fp)
+    return a.ctypes.data_as(fp)
+
+
+def dptr(a):
+    """double* of a float64/complex128 numpy array or of a device tensor."""
+    if hasattr(a, "data_ptr"):
+        return ctypes.cast(ctypes.c_void_p(a.data_ptr()), dp)
     return a.ctypes.data_as(dp)

@@ -27,14 +27,19 @@ using vk::cplx;
 using vk::Geo;
 
 namespace {
-
 thread_local std::string g_err;
+}
 
-int fail(int code, const std::string& msg)
+// shared with the other translation units of the library (vk_last_error)
+int vk_set_error(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
+
+namespace {
+
+int fail(int code, const std::string& msg) { return vk_set_error(code, msg); }
 
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
@@ -80,6 +85,7 @@ struct vk_gpe {
     int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
     int kv_valid = 0;
     int have_part = 0;
+    int np_cap = 0;     // particle buffers are sized for this many particles
     int rs_part = 0;   // particle potential and force evaluated in real space inside P3
     // diagnostics
     cplx* X[7] = {};
@@ -144,15 +150,20 @@ void ev_end(vk_gpe* h, int kind)
 }
 
 std::mutex g_lds_mu;
-std::set<const void*> g_lds_done;
+std::map<const void*, size_t> g_lds_max;   // dynamic-LDS limit set per kernel so far
 
 template <class Kern>
 void set_lds(Kern k, size_t bytes)
 {
+    // the limit only grows: a kernel whose LDS depends on a runtime size
+    // (P3 with np particles) may be launched larger than at its first use
     if (bytes <= 65536) return;
     std::lock_guard<std::mutex> lk(g_lds_mu);
-    if (g_lds_done.insert((const void*)k).second)
+    size_t& cur = g_lds_max[(const void*)k];
+    if (bytes > cur) {
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        cur = bytes;
+    }
 }
 
 template <class C, class Kern, class... A>
@@ -384,6 +395,7 @@ void join_particles(vk_gpe* h)
 
 int ens1_step(vk_gpe* h);
 int make_single_table(vk_gpe* h);
+vk::EnsReal ens_entry(const vk_gpe* h);
 
 int rk4_step(vk_gpe* h)
 {
@@ -631,6 +643,7 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
     h->geo.Pp = (M + 1 + TC - 1) / TC;
     h->geo.Pn = (M + TC - 1) / TC;
     h->rs_part = use_real_space(*p);
+    h->np_cap = p->np;
     if (h->geo.Pp + h->geo.Pn > h->NP) { h->geo.Pp = h->NP; h->geo.Pn = 0; }
     auto cleanup = [&](int rc) { vk_gpe_destroy(h); return rc; };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
@@ -752,14 +765,17 @@ int vk_gpe_set_psi(vk_gpe* h, const double* psi, int flags)
     if (!h || !psi) return fail(VK_ERR_ARG, "set_psi: null argument");
     int rc = need_x0(h);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(h->X[0], psi, f2(h) * sizeof(cplx), hipMemcpyHostToDevice));
+    // stream-ordered: a device-to-device hipMemcpy does not block the host and
+    // is not ordered against this non-blocking stream
+    HIPCHK(hipMemcpyAsync(h->X[0], psi, f2(h) * sizeof(cplx), hipMemcpyDefault, h->stream));
     return set_from_x0(h, flags);
 }
 
 int vk_gpe_set_kpsi(vk_gpe* h, const double* kpsi)
 {
     if (!h || !kpsi) return fail(VK_ERR_ARG, "set_kpsi: null argument");
-    HIPCHK(hipMemcpy(h->K0, kpsi, f2(h) * sizeof(cplx), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(h->K0, kpsi, f2(h) * sizeof(cplx), hipMemcpyDefault, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     h->hstate = 0;
     return 0;
 }
@@ -774,11 +790,13 @@ int vk_gpe_get_psi(vk_gpe* h, double* psi, double* kpsi)
         if ((rc = row_inv(h, h->Hpsi, h->X[0], h->hstate == 2 ? 1 : 0, 0, scale_of(h)))) return rc;
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy(psi, h->X[0], f2(h) * sizeof(cplx), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(psi, h->X[0], f2(h) * sizeof(cplx), hipMemcpyDefault, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     if (kpsi) {
         HIPCHK(hipStreamSynchronize(h->stream));
-        HIPCHK(hipMemcpy(kpsi, h->K0, f2(h) * sizeof(cplx), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(kpsi, h->K0, f2(h) * sizeof(cplx), hipMemcpyDefault, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     return 0;
 }
@@ -849,6 +867,19 @@ int vk_gpe_diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal)
     return diagnostics(h, scal, spec, pscal);
 }
 
+// particle_potential (src/gpe_particle.f90:3-90): V of one particle at the
+// origin on the grid, forward transform, dealias -> Vhat
+static int particle_potential(vk_gpe* h)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    const double L = 2.0 * pi, dx = L / h->N, c1 = 2 * pi / (2.0 * L);
+    hipLaunchKernelGGL(vk::k_pot_x, dim3(1024), dim3(256), 0, h->stream, h->X[0], h->N, dx, c1, h->p.V0, h->p.d);
+    int rc = fwd2d(h, h->X[0], h->Hd, h->Vhat, 0, 1);
+    if (rc) return rc;
+    h->have_part = 1;
+    return 0;
+}
+
 int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel, const double* fext)
 {
     if (!h) return fail(VK_ERR_ARG, "particles_set: null handle");
@@ -859,16 +890,46 @@ int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel, const 
     if (pos) HIPCHK(hipMemcpy(h->pst, pos, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
     if (vel) HIPCHK(hipMemcpy(h->pst + 2 * np, vel, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
     if (fext) HIPCHK(hipMemcpy(h->pst + 4 * np, fext, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
-    if (!h->have_part) {
-        // particle_potential (src/gpe_particle.f90:3-90): V on the grid,
-        // forward transform, dealias -> Vhat
-        const double pi = 4.0 * std::atan(1.0);
-        const double L = 2.0 * pi, dx = L / h->N, c1 = 2 * pi / (2.0 * L);
-        hipLaunchKernelGGL(vk::k_pot_x, dim3(1024), dim3(256), 0, h->stream, h->X[0], h->N, dx, c1, h->p.V0, h->p.d);
-        if ((rc = fwd2d(h, h->X[0], h->Hd, h->Vhat, 0, 1))) return rc;
-        h->have_part = 1;
-    }
+    if (!h->have_part && (rc = particle_potential(h))) return rc;
     h->kv_valid = 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vk_gpe_set_dt(vk_gpe* h, double dt)
+{
+    if (!h || !(dt > 0.0) || !std::isfinite(dt)) return fail(VK_ERR_ARG, "set_dt: dt must be positive and finite");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->p.dt = dt;
+    if (h->e1R) {
+        const vk::EnsReal x = ens_entry(h);
+        HIPCHK(hipMemcpy(h->e1R, &x, sizeof(x), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int vk_gpe_set_obstacles(vk_gpe* h, int np, const double* pos, double V0, double d)
+{
+    if (!h) return fail(VK_ERR_ARG, "set_obstacles: null handle");
+    if (np < 0 || np > h->np_cap)
+        return fail(VK_ERR_ARG, "set_obstacles: np must be in [0, np of the params the handle was created with]");
+    if (np > 0 && (!pos || !(d > 0.0) || !std::isfinite(V0)))
+        return fail(VK_ERR_ARG, "set_obstacles: positions, d > 0 and a finite V0 are required");
+    int rc = need_x0(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->p.np = np;
+    h->p.V0 = V0;
+    h->p.d = d;
+    h->rs_part = use_real_space(h->p);
+    h->have_part = 0;
+    h->kv_valid = 0;
+    if (np == 0) return 0;
+    std::vector<double> st(30 * (size_t)np + 8, 0.0);
+    std::copy(pos, pos + 2 * np, st.begin());
+    HIPCHK(hipMemcpyAsync(h->pst, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice, h->stream));
+    if ((rc = particle_potential(h))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipGetLastError());
     return 0;

@@ -54,7 +54,7 @@ class GPE:
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and nat is not None:
             nat.lib().vk_gpe_destroy(self.h)
             self.h = None
 
@@ -148,6 +148,32 @@ class GPE:
         fext = np.zeros((n, 2)) if fext is None else np.ascontiguousarray(fext, dtype=np.float64).reshape(n, 2)
         nat.check(nat.lib().vk_gpe_particles_set(self.h, nat.ptr(pos), nat.ptr(vel), nat.ptr(fext)))
 
+    def set_obstacles(self, pos, V0, d):
+        """Static obstacles for ARGLE: np = len(pos) particles (at most the np
+        the handle was created with) at pos (np, 2), amplitude V0 and width d
+        (absolute units), velocities cleared."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+        n = pos.shape[0]
+        nat.check(nat.lib().vk_gpe_set_obstacles(self.h, n, nat.ptr(pos) if n else None, float(V0), float(d)))
+        self.params.np = n
+        self.params.V0 = float(V0)
+        self.params.d = float(d)
+
+    def set_dt(self, dt):
+        """Replace dt_fix of the live state."""
+        nat.check(nat.lib().vk_gpe_set_dt(self.h, float(dt)))
+        self.params.dt = float(dt)
+
+    def set_psi_from(self, psi, sync=True):
+        """set_psi from a complex128 [y, x] array that may live on the GPU
+        (anything with data_ptr(), e.g. a torch tensor) or on the host."""
+        nat.check(nat.lib().vk_gpe_set_psi(self.h, nat.dptr(psi), 1 if sync else 0))
+
+    def psi_to(self, psi):
+        """Current psi as complex128 [y, x] into a host array or a device
+        tensor (anything with data_ptr())."""
+        nat.check(nat.lib().vk_gpe_get_psi(self.h, nat.dptr(psi), None))
+
     def particles(self):
         n = self.params.np
         pos, vel, frc = np.zeros((n, 2)), np.zeros((n, 2)), np.zeros((n, 2))
@@ -190,7 +216,7 @@ class Ensemble:
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and nat is not None:
             nat.lib().vk_ens_destroy(self.h)
             self.h = None
 
