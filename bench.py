@@ -14,7 +14,9 @@ per-kernel roofline of the dominant pass (algorithmic bytes / measured
 average duration, against the 8 TB/s HBM peak), and the CPU baseline: the
 plain-C oracle (same algorithm as the reference, 16 unfused 2D FFTs per step
 plus the per-particle transforms) timed on a bounded sample of the same
-workload on the host cores.
+workload on the host cores; the batched ensemble; and ARGLE time-to-1e-10
+at 512^2 with the learned warm start against plain ARGLE from the Pade
+ansatz (and against the Thomas-Fermi start).
 """
 import argparse
 import json
@@ -160,6 +162,46 @@ def ensemble_bench(G, torch, steps, warmup):
                                      for m in range(3)}}
 
 
+def argle_bench(G, torch, n_cfg, plain_cap):
+    """ARGLE time-to-1e-10 at 512^2: warm start (network + ARGLE) vs plain
+    ARGLE from the Pade product ansatz vs the Thomas-Fermi start, on held-out
+    seeded configurations (16-128 vortices, 0-8 obstacles, V0 in [2, 20] g).
+    Wall time from the first call (input planes / ansatz) to the step that
+    meets the residual; plain runs are capped at plain_cap steps."""
+    from quantumvortices_amd import warmstart as W
+    fno = W.FNO.load(os.path.join(ROOT, "quantumvortices_amd", "data", "fno_argle_512.npz"), batch=1)
+    g = G.GPE(W.solver_params(512, W.DT_MAX))
+    # untimed first pass: kernel loading and LDS limits
+    W.warm_start(g, fno, W.random_config(899999), torch.empty((512, 512), dtype=torch.complex128, device="cuda"))
+    rows = W.compare(fno, g, [W.random_config(900000 + k) for k in range(n_cfg)], plain_cap=plain_cap,
+                     plain_check=2000, log=log)
+    fno.close()
+    g.close()
+    conv = [r for r in rows if r["plain_converged"]]
+    gm = lambda xs: float(math.exp(sum(math.log(x) for x in xs) / len(xs))) if xs else None
+    return {
+        "workload": "512^2 ARGLE to residual 1e-10, dt 0.08 (lower for overlapping obstacles), "
+                    "held-out seeds 900000+, network trained on %d solver-generated ground states"
+                    % fno.info.get("samples", 0),
+        "warm_ms": [round(1e3 * r["warm_sec"], 2) for r in rows],
+        "tf_ms": [round(1e3 * r["tf_sec"], 2) for r in rows],
+        "plain_ms": [round(1e3 * r["plain_sec"], 1) for r in rows],
+        "warm_steps": [r["warm_steps"] for r in rows],
+        "tf_steps": [r["tf_steps"] for r in rows],
+        "plain_steps": [r["plain_steps"] for r in rows],
+        "plain_converged": [r["plain_converged"] for r in rows],
+        "speedup_vs_plain": round(gm([r["plain_sec"] / r["warm_sec"] for r in rows]), 1),
+        "speedup_vs_plain_is_lower_bound": len(conv) < len(rows),
+        "speedup_vs_tf": round(gm([r["tf_sec"] / r["warm_sec"] for r in rows]), 3),
+        "omega_warm_minus_plain_rel": [float("%.3g" % ((r["warm_omega"] - r["plain_omega"]) / abs(r["plain_omega"])))
+                                       for r in rows],
+        "omega_warm_minus_tf_rel": [float("%.3g" % ((r["warm_omega"] - r["tf_omega"]) / abs(r["tf_omega"])))
+                                    for r in rows],
+        "surrogate_raw_rel_err": [float("%.3g" % r["raw_rel_err"]) for r in rows],
+        "tf_raw_rel_err": [float("%.3g" % r["tf_raw_rel_err"]) for r in rows],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +213,9 @@ def main():
     ap.add_argument("--no-pure", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-argle", action="store_true")
+    ap.add_argument("--argle-configs", type=int, default=2)
+    ap.add_argument("--argle-plain-cap", type=int, default=400000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the single-field path shards as "
@@ -225,6 +270,12 @@ def main():
         ens = ensemble_bench(G, torch, max(10, args.steps // 4), 3)
         log("bench: ensemble", ens)
 
+    argle = None
+    if not args.no_argle:
+        log("bench: ARGLE warm start vs plain at 512^2")
+        argle = argle_bench(G, torch, args.argle_configs, args.argle_plain_cap)
+        log("bench: argle", argle)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -254,6 +305,7 @@ def main():
         "effective_GBps_vs_53_transfer_step": round(step_bytes_ref * steps_per_s / 1e9, 1),
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "argle_warm_start": argle,
     }
     print(json.dumps(out), flush=True)
 
