@@ -26,6 +26,12 @@
 using vk::cplx;
 using vk::Geo;
 
+// single-field handles of at least this size keep x-space psi rows between
+// P1 and P3 (one N^2 field more; P3 then runs two transforms instead of three)
+#ifndef VK_XPSI_MIN_N
+#define VK_XPSI_MIN_N 2048
+#endif
+
 namespace {
 thread_local std::string g_err;
 }
@@ -80,6 +86,8 @@ struct vk_gpe {
     cplx* tw = nullptr;
     // state and RK4/ARGLE work fields
     cplx *K0 = nullptr, *K1 = nullptr, *K2 = nullptr, *Hpsi = nullptr, *Hrho = nullptr, *HN = nullptr;
+    cplx* Xs = nullptr;   // x-space psi rows written by P1 and read by P3 (xpsi)
+    int xpsi = 0;
     cplx *HV = nullptr, *KV = nullptr, *Krho = nullptr, *Vhat = nullptr, *EX = nullptr, *EY = nullptr;
     double *fpart = nullptr, *pst = nullptr, *part = nullptr, *dsum = nullptr;
     int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
@@ -182,10 +190,13 @@ void row_launch(vk_gpe* h, int kind, Kern k, A... args)
     row_launch_x<C>(h, kind, 0, k, args...);
 }
 
-// P3 keeps S doubles per thread in LDS besides the transform regions
+// P3 keeps S doubles per thread in LDS besides the transform regions (in
+// registers instead with xpsi and no real-space particles), then the
+// particle positions
 template <class C> size_t p3_extra(const vk_gpe* h)
 {
-    return (size_t)C::S * C::ROWT * sizeof(double) + (size_t)2 * std::max(h->p.np, 1) * sizeof(double);
+    const bool park = !h->xpsi || (h->have_part && h->p.np > 0 && h->rs_part);
+    return (park ? (size_t)C::S * C::ROWT * sizeof(double) : 0) + (size_t)2 * std::max(h->p.np, 1) * sizeof(double);
 }
 
 template <class C, class Kern, class... A>
@@ -414,12 +425,20 @@ int rk4_step(vk_gpe* h)
         a.omega = h->p.omega;
         for (int stage = 1; stage <= 4; ++stage) {
             if (parts && !rs) particle_kv(h, Kp);
-            row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc);
+            if (h->xpsi)
+                row_launch<C>(h, PK_P1, vk::k_p1x<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc,
+                              h->Xs);
+            else
+                row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc);
             col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                           (const cplx*)h->tw, h->geo, (parts && !rs) ? 3 : 0);
             if (parts && !rs) particle_forces(h, h->Krho, Kp, 1, stage);
             if (rs) join_particles(h);   // positions from the previous stage
-            {
+            if (h->xpsi) {
+                auto k3 = !parts ? vk::k_p3x<C, 0> : (rs ? vk::k_p3x<C, 6> : vk::k_p3x<C, 1>);
+                row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV,
+                                (const cplx*)h->Xs, h->HN, (const cplx*)h->tw, h->geo, 1, h->p.g * sc, sc, 1.0, pa);
+            } else {
                 auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 6> : vk::k_p3<C, 1>);
                 row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
                               h->HN, (const cplx*)h->tw, h->geo, 1, 1, h->p.g * sc, sc, sc, 1.0, pa);
@@ -449,10 +468,18 @@ int argle_step(vk_gpe* h, int want_res)
             particle_kv(h, C::TC * h->geo.Pp);
             h->kv_valid = 1;
         }
-        row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc);
+        if (h->xpsi)
+            row_launch<C>(h, PK_P1, vk::k_p1x<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc,
+                          h->Xs);
+        else
+            row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc);
         col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                       (const cplx*)h->tw, h->geo, (parts && !rs) ? 1 : 0);
-        {
+        if (h->xpsi) {
+            auto k3 = !parts ? vk::k_p3x<C, 0> : (rs ? vk::k_p3x<C, 2> : vk::k_p3x<C, 1>);
+            row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Xs,
+                            h->HN, (const cplx*)h->tw, h->geo, 0, -h->p.g * sc, -sc, -1.0, pa);
+        } else {
             auto k3 = !parts ? vk::k_p3<C, 0> : (rs ? vk::k_p3<C, 2> : vk::k_p3<C, 1>);
             row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi, h->HN,
                           (const cplx*)h->tw, h->geo, 0, 0, -h->p.g * sc, -sc, sc, -1.0, pa);
@@ -654,6 +681,8 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         if (hipMalloc(b, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (fields)"));
         hipMemset(*b, 0, fb);
     }
+    h->xpsi = N >= VK_XPSI_MIN_N;
+    if (h->xpsi && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
     if (hipMalloc(&h->tw, sizeof(cplx) * N) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc tw"));
     // twiddles w_k = exp(-2 pi i k/N): long-double sin/cos on the first
     // octant, exact quadrant/octant symmetries elsewhere
@@ -711,9 +740,9 @@ void vk_gpe_destroy(vk_gpe* h)
 {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
-    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd};
+    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd, h->Xs};
     for (auto f : fields) if (f) hipFree(f);
-    if (h->p.np > 0) {
+    if (h->np_cap > 0) {
         cplx* pf[] = {h->HV, h->KV, h->Krho, h->Vhat, h->EX, h->EY};
         for (auto f : pf) if (f) hipFree(f);
         if (h->fpart) hipFree(h->fpart);
@@ -1051,9 +1080,9 @@ double vk_gpe_pass_bytes(vk_gpe* h, int kind)
             if (!masked_host(kx, ky, h->N)) ++disk;
     const double D = 16.0 * (double)disk;
     switch (kind) {
-    case PK_P1: return F * (fr + fp);
+    case PK_P1: return F * (fr + fp + (h->xpsi ? 1.0 : 0.0));   // + psi rows written for P3
     case PK_P2: return F * 2.0 * fp;
-    case PK_P3: return F * (fp + fr + fr);
+    case PK_P3: return F * (fp + (h->xpsi ? 1.0 : fr) + fr);    // psi rows instead of Hpsi
     case PK_P4: return F * (fr + fr) + D * (3 + 5 + 5 + 4) / 4.0;
     default: return 0.0;
     }

@@ -180,11 +180,13 @@ __global__ void k_dealias(cplx* __restrict__ K, int N)
 // ----------------------------------------------------------------------------
 
 // P1 (row): psi row from Hpsi (inverse along x, times 1/N^2), rho = |psi|^2,
-// forward along x, keep kx >= 0 retained panels (rho is real).
-template <class C>
+// forward along x, keep kx >= 0 retained panels (rho is real).  XPSI: also
+// store the psi row in x-space (Xpsi[y*N + x]) for P3, which then needs no
+// transform of its own for psi.
+template <class C, bool XPSI = false>
 __device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
-                                                double scale)
+                                                double scale, cplx* __restrict__ Xpsi = nullptr)
 {
     VK_ROW_SETUP(C)
     cplx v[S];
@@ -194,6 +196,11 @@ __device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restr
         v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
     }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    if constexpr (XPSI) {
+        cplx* xr = Xpsi + (size_t)y * N;
+#pragma unroll
+        for (int s = 0; s < S; ++s) xr[t + T * s] = rmul(scale, conjc(v[s]));
+    }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const double a = scale * v[s].x, b = scale * v[s].y;   // conj does not change |.|
@@ -213,6 +220,14 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ 
                                                 double scale)
 {
     p1_body<C>(blockIdx.x, gridDim.x, Hpsi, Hrho, tw, g, psi_pmode, scale);
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1x(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                 double scale, cplx* __restrict__ Xpsi)
+{
+    p1_body<C, true>(blockIdx.x, gridDim.x, Hpsi, Hrho, tw, g, psi_pmode, scale, Xpsi);
 }
 
 // P2 (column, kx >= 0 retained panels): forward along y, dealias
@@ -341,20 +356,24 @@ __device__ __forceinline__ void pcoord(double a, double& X, double& dX)
 // PM (particle mode, compile time): 0 none; 1 spectral potential (HV);
 // 2 real-space potential; 6 real-space potential + force partial sums.
 // vsign: +1 RK4, -1 ARGLE.
-template <class C, int PM>
+// XPSI: psi comes from the x-space rows P1 stored (already scaled), so the
+// factor f stays in registers and no transform of Hpsi runs here.
+template <class C, int PM, bool XPSI = false>
 __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
                                                 const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
                                                 int store_pmode, double gs, double vs, double scale,
-                                                double vsign, P3Part pa)
+                                                double vsign, P3Part pa, const cplx* __restrict__ Xpsi = nullptr)
 {
     constexpr int flags = PM;
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
-    // particle positions staged in LDS (after the f slots) during the first
-    // transform; read back as broadcasts in the particle loop
-    double* qlds = lds + TR * C::ROWRS + S * C::ROWT;
+    // f in registers (XPSI without real-space particles) or parked in LDS
+    // slots after the transform regions; particle positions staged after
+    // those during the first transform, read back as broadcasts
+    constexpr bool FREG = XPSI && (flags & 2) == 0;
+    double* qlds = lds + TR * C::ROWRS + (FREG ? 0 : S * C::ROWT);
     if constexpr ((flags & 2) != 0) {
         for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
     }
@@ -379,12 +398,14 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     // The real factor f = g rho_d + V goes to thread-private LDS slots (after
     // the transform regions) and waits there while psi is transformed.
     double* fl = lds + TR * C::ROWRS;
+    double fr_[FREG ? S : 1];
+#define VK_F(s) (FREG ? fr_[FREG ? (s) : 0] : fl[(s) * C::ROWT + threadIdx.x])
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         double fv;
         if constexpr ((flags & 1) != 0) fv = gs * v[s].x + vs * (-v[s].y);
         else fv = gs * v[s].x;
-        fl[s * C::ROWT + threadIdx.x] = fv;
+        VK_F(s) = fv;
     }
     if constexpr ((flags & 2) != 0) {
         // real-space particle potential added into the slots; force partials
@@ -427,7 +448,7 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
                         double X, dX;
                         pcoord(pwrap((t + T * s) * pa.dx - qx), X, dX);
                         const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
-                        fl[s * C::ROWT + threadIdx.x] += vsign * e;
+                        VK_F(s) += vsign * e;
                         fr[0] += rho[s] * (-e * X * dX * invd2);
                         fr[1] += rho[s] * (-e * Y * dY * invd2);
                     }
@@ -442,17 +463,24 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
             }
         }
     }
+    if constexpr (XPSI) {
+        const cplx* xr = Xpsi + (size_t)y * N;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int pp = VK_PANEL(s);
-        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
-    }
-    fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+        for (int s = 0; s < S; ++s) v[s] = rmul(VK_F(s), xr[t + T * s]);
+    } else {
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const cplx ps = rmul(scale, conjc(v[s]));
-        v[s] = rmul(fl[s * C::ROWT + threadIdx.x], ps);
+        for (int s = 0; s < S; ++s) {
+            const int pp = VK_PANEL(s);
+            v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
+        }
+        fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const cplx ps = rmul(scale, conjc(v[s]));
+            v[s] = rmul(VK_F(s), ps);
+        }
     }
+#undef VK_F
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -469,6 +497,16 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
                                                 double vsign, P3Part pa)
 {
     p3_body<C, PM>(blockIdx.x, gridDim.x, Hrhod, HV, Hpsi, HN, tw, g, psi_pmode, store_pmode, gs, vs, scale, vsign, pa);
+}
+
+template <class C, int PM>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3x(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+                                                 const cplx* __restrict__ Xpsi, cplx* __restrict__ HN,
+                                                 const cplx* __restrict__ tw, Geo g, int store_pmode, double gs,
+                                                 double vs, double vsign, P3Part pa)
+{
+    p3_body<C, PM, true>(blockIdx.x, gridDim.x, Hrhod, HV, nullptr, HN, tw, g, 0, store_pmode, gs, vs, 1.0, vsign,
+                         pa, Xpsi);
 }
 
 struct RKArgs {

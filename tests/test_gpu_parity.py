@@ -103,6 +103,55 @@ def test_argle_vs_oracle(G, oracle_mod, N, steps):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["rk4", "rk4_particles", "argle_obstacles"])
+def test_large_n_vs_oracle(G, oracle_mod, case):
+    """N = 2048: the single-field path keeps x-space psi rows between the
+    row passes (P3 reads them instead of transforming Hpsi again)."""
+    N = 2048
+    say("case large-N %s N=%d" % (case, N))
+    O = oracle_mod
+    psi0 = _ic(G, N, nv_lattice=4)
+    # next to vortex cores, where the density gradient gives O(1) forces
+    xs, ys, _ = G.lattice_vortices(4)
+    pos = np.stack([xs[[0, 5, 10, 15]] + 0.04, ys[[0, 5, 10, 15]] - 0.03], axis=1)
+    if case == "rk4":
+        dt, steps, kw, okw = 1e-3, 2, {}, {}
+    elif case == "rk4_particles":
+        dt, steps = 1e-3, 2
+        kw = dict(gp_N_obj=4, ppotV0=10.0, ppotd=2.0, gp_mass_part=5.0)
+        okw = dict(np_=4, V0_in=10.0, d_in=2.0, mass=5.0)
+    else:
+        dt, steps = 2e-2, 3
+        kw = dict(gp_N_obj=2, ppotV0=10.0, ppotd=2.0, gp_mass_part=1.0)
+        okw = dict(np_=2, V0_in=10.0, d_in=2.0, mass=1.0)
+        pos = pos[:2]
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5, **kw))
+    st = O.State(O.make_params(N, dt, 2.5, **okw))
+    sync = case != "argle_obstacles"
+    g.set_psi(psi0, sync=sync)
+    st.set_psi(psi0, sync=sync)
+    if kw:
+        g.initialize_particles(pos)
+        st.part(0)[...] = pos
+        st.particle_potential()
+    if case == "argle_obstacles":
+        r, ro = g.ARGLE(nsteps=steps), st.argle(steps)
+        assert abs(r - ro) <= 1e-9 * abs(ro)
+    else:
+        g.gpe_rk4(steps)
+        st.rk4(steps)
+    e = rel_l2(g.kpsi, st.kpsi)
+    say("  rel-L2 kpsi %.3e" % e)
+    assert e < TOL
+    if case == "rk4_particles":
+        gp, gv, _ = g.particles()
+        say("  |vel| %.3e rel-L2 pos %.3e vel %.3e" % (np.abs(gv).max(), rel_l2(gp, st.part(0)),
+                                                      rel_l2(gv, st.part(1))))
+        assert np.abs(gv).max() > 1e-12           # the particles do move
+        assert rel_l2(gp, st.part(0)) < TOL and rel_l2(gv, st.part(1)) < 1e-10
+    g.close()
+
+
 @pytest.mark.parametrize("N", [64, 128])
 def test_diagnostics_vs_oracle(G, oracle_mod, N):
     say("case diagnostics N=%d" % N)
