@@ -162,6 +162,23 @@ def ensemble_bench(G, torch, steps, warmup):
                                      for m in range(3)}}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk4_4096_traffic.txt")
+PASS_KERNEL = {"p1_rho_row": "vk::k_p1x<4096 >", "p2_dealias_col": "vk::k_p2<4096 >",
+               "p3_nonlin_row": "vk::k_p3x<4096, 6>", "p4_rk4_col": "vk::k_p4<4096 >"}
+
+
+def measured_traffic(pass_name, N):
+    """HBM bytes per launch of a pass at 4096^2 with particles, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
+    device copy); None for other sizes or when the summary is missing."""
+    if N != 4096 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith(PASS_KERNEL.get(pass_name, "?") + " "):
+            return float(line.split()[-2 if len(line.split()) > 7 else -1]) * 1e6
+    return None
+
+
 def argle_bench(G, torch, n_cfg, plain_cap):
     """ARGLE time-to-1e-10 at 512^2: warm start (network + ARGLE) vs plain
     ARGLE from the Pade product ansatz vs the Thomas-Fermi start, on held-out
@@ -256,7 +273,7 @@ def main():
     dom = max(passes, key=lambda k: pt[k])
     achieved = pb[dom] / (pt[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, N),
             "bytes_per_launch": pb[dom], "avg_ms": pt[dom]}
     per_pass = {k: {"ms": round(pt[k], 4), "GB/s": round(pb[k] / (pt[k] * 1e-3) / 1e9, 1) if pt[k] > 0 else None}
                 for k in passes}
