@@ -12,6 +12,10 @@
 #pragma once
 #include "vk_fft.hpp"
 
+#ifndef VK_STAGGER
+#define VK_STAGGER 32
+#endif
+
 namespace vk {
 
 template <int N_, int S_, int TC_, int TR_, bool SPLIT_>
@@ -26,11 +30,11 @@ struct CfgT {
     static constexpr int LDSW = Lds<N_>::words * (SPLIT_ ? 1 : 2);  // doubles per transform
     static constexpr int COLT = TC_ * T;  // column-pass threads
     static constexpr int ROWT = TR_ * T;  // row-pass threads
-    // LDS stride between the transforms of one workgroup: staggered by 16/G
-    // words so that transforms interleaved across lanes start on different
-    // bank pairs
-    static constexpr int COLRS = LDSW + 16 / TC_;
-    static constexpr int ROWRS = LDSW + 16 / TR_;
+    // LDS stride between the transforms of one workgroup: staggered by
+    // VK_STAGGER/G doubles so that the G transforms interleaved across the
+    // lanes of a wave start on disjoint bank pairs
+    static constexpr int COLRS = LDSW + VK_STAGGER / TC_;
+    static constexpr int ROWRS = LDSW + VK_STAGGER / TR_;
     static_assert(T % TC == 0, "TC must divide T");
     static_assert(T >= 1, "N/S >= 1");
 };
