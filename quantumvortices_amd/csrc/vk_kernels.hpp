@@ -440,18 +440,35 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
             if (fabs(dr) <= Wg) {
                 double Y, dY;
                 pcoord(pwrap(y * pa.dx - qy), Y, dY);
+                // points of this thread inside the window (cheap test for
+                // all S), then the transcendental work once per such point,
+                // in increasing s (a single point when the window is
+                // narrower than the thread stride T)
+                unsigned mask = 0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     double dc = (t + T * s) - cx;
                     dc -= N * rint(dc * invN);
-                    if (fabs(dc) <= Wg) {
-                        double X, dX;
-                        pcoord(pwrap((t + T * s) * pa.dx - qx), X, dX);
-                        const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
-                        VK_F(s) += vsign * e;
-                        fr[0] += rho[s] * (-e * X * dX * invd2);
-                        fr[1] += rho[s] * (-e * Y * dY * invd2);
+                    if (fabs(dc) <= Wg) mask |= 1u << s;
+                }
+                while (mask) {
+                    const int sw = __builtin_ctz(mask);
+                    mask &= mask - 1;
+                    double rs = 0.0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) rs = (s == sw) ? rho[s] : rs;
+                    double X, dX;
+                    pcoord(pwrap((t + T * sw) * pa.dx - qx), X, dX);
+                    const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
+                    if constexpr (FREG) {
+#pragma unroll
+                        for (int s = 0; s < S; ++s)
+                            if (s == sw) fr_[FREG ? s : 0] += vsign * e;
+                    } else {
+                        fl[sw * C::ROWT + threadIdx.x] += vsign * e;
                     }
+                    fr[0] += rs * (-e * X * dX * invd2);
+                    fr[1] += rs * (-e * Y * dY * invd2);
                 }
             }
             if constexpr ((flags & 4) != 0) {
