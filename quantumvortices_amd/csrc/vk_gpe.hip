@@ -192,11 +192,13 @@ void row_launch(vk_gpe* h, int kind, Kern k, A... args)
 
 // P3 keeps S doubles per thread in LDS besides the transform regions (in
 // registers instead with xpsi and no real-space particles), then the
-// particle positions
+// particle positions and a one-bit-per-particle hit mask
 template <class C> size_t p3_extra(const vk_gpe* h)
 {
     const bool park = !h->xpsi || (h->have_part && h->p.np > 0 && h->rs_part);
-    return (park ? (size_t)C::S * C::ROWT * sizeof(double) : 0) + (size_t)2 * std::max(h->p.np, 1) * sizeof(double);
+    const size_t np = (size_t)std::max(h->p.np, 1);
+    return (park ? (size_t)C::S * C::ROWT * sizeof(double) : 0) + 2 * np * sizeof(double) +
+           (np + 31) / 32 * sizeof(unsigned);
 }
 
 template <class C, class Kern, class... A>

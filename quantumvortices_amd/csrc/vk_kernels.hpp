@@ -374,8 +374,12 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     // those during the first transform, read back as broadcasts
     constexpr bool FREG = XPSI && (flags & 2) == 0;
     double* qlds = lds + TR * C::ROWRS + (FREG ? 0 : S * C::ROWT);
+    // hit mask (one bit per particle whose window reaches this row block)
+    // after the positions
+    unsigned* hmask = reinterpret_cast<unsigned*>(qlds + 2 * pa.np);
     if constexpr ((flags & 2) != 0) {
         for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
+        for (int i = threadIdx.x; i < (pa.np + 31) / 32; i += blockDim.x) hmask[i] = 0u;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -416,24 +420,32 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
         const int yb = blk * TR;
         const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
         const double idx = 1.0 / pa.dx, Wg = pa.W * idx, invN = 1.0 / N;
-        for (int pp = 0; pp < pa.np; ++pp) {
-            const double qx = qlds[2 * pp], qy = qlds[2 * pp + 1];
-            const double cx = qx * idx, cy = qy * idx;   // centre in grid units
-            // window test in grid units (multiplications only): minimum-image
-            // row distance of the workgroup's rows to the particle
+        // window test, one thread per particle, in grid units
+        // (multiplications only): minimum-image row distance of the
+        // workgroup's rows to the particle; particles that miss get zero
+        // force partials here
+        for (int pp = threadIdx.x; pp < pa.np; pp += blockDim.x) {
+            const double cy = qlds[2 * pp + 1] * idx;
             bool hit = false;
             for (int r2 = 0; r2 < TR; ++r2) {
                 double dr = (yb + r2) - cy;
                 dr -= N * rint(dr * invN);
                 hit |= fabs(dr) <= Wg;
             }
-            if (!hit) {
-                if (((flags & 4) != 0) && threadIdx.x == 0) {
-                    pa.ppart[((size_t)blk * pa.np + pp) * 2] = 0.0;
-                    pa.ppart[((size_t)blk * pa.np + pp) * 2 + 1] = 0.0;
-                }
-                continue;
+            if (hit) {
+                atomicOr(&hmask[pp >> 5], 1u << (pp & 31));
+            } else if constexpr ((flags & 4) != 0) {
+                pa.ppart[((size_t)blk * pa.np + pp) * 2] = 0.0;
+                pa.ppart[((size_t)blk * pa.np + pp) * 2 + 1] = 0.0;
             }
+        }
+        __syncthreads();
+        // hit particles in increasing order (uniform over the workgroup)
+        for (int w = 0; w < (pa.np + 31) / 32; ++w)
+        for (unsigned m = hmask[w]; m != 0u; m &= m - 1u) {
+            const int pp = 32 * w + __builtin_ctz(m);
+            const double qx = qlds[2 * pp], qy = qlds[2 * pp + 1];
+            const double cx = qx * idx, cy = qy * idx;   // centre in grid units
             double fr[2] = {0.0, 0.0};
             double dr = y - cy;
             dr -= N * rint(dr * invN);
