@@ -183,6 +183,107 @@ def test_diagnostics_vs_oracle(G, oracle_mod, N):
     g.close()
 
 
+def test_reference_cpu_case_128(G, oracle_mod):
+    """The reference's own CPU-runnable configuration end to end: 128^2,
+    g = 1, xi = 2.5 dx, 4x4 lattice of alternating vortices (Pade cores),
+    RK4 dt = 1e-3 for 1000 steps with the diagnostics every 100 steps.  At
+    every checkpoint the field, the energies and all six spectra match the
+    oracle, and norm and energy drift match it too."""
+    N, dt = 128, 1e-3
+    say("case reference 128^2 x 1000 steps")
+    O = oracle_mod
+    psi0 = _ic(G, N)
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5))
+    g.set_psi(psi0, sync=True)
+    st = O.State(O.make_params(N, dt, 2.5))
+    st.set_psi(psi0, sync=True)
+    first = None
+    for blk in range(10):
+        g.gpe_rk4(100)
+        st.rk4(100)
+        d = g.diagnostics_gpe()
+        scal, spec, _ = st.diagnostics()
+        e_psi = rel_l2(g.kpsi, st.kpsi)
+        e_spec = max(rel_l2(d["spec"][i], spec[i]) for i in range(6))
+        say("  step %4d rel-L2 kpsi %.2e  worst spectrum %.2e  etot %.12e  norm %.12e"
+            % (100 * (blk + 1), e_psi, e_spec, d["etot"], d["norm"]))
+        assert e_psi < TOL and e_spec < TOL
+        for i in (0, 1, 2, 3, 4):
+            assert abs(d["scal"][i] - scal[i]) <= 1e-11 * abs(scal[i])
+        if first is None:
+            first = (d["norm"], d["etot"], scal[4], scal[3])
+    # drift over the run is the scheme's, identical on both sides
+    dn_g, dn_o = d["norm"] - first[0], scal[4] - first[2]
+    de_g, de_o = d["etot"] - first[1], scal[3] - first[3]
+    say("  drift after 900 steps: norm %.3e / %.3e  etot %.3e / %.3e" % (dn_g, dn_o, de_g, de_o))
+    assert abs(dn_g - dn_o) <= 1e-12 * first[0] and abs(de_g - de_o) <= 1e-12 * abs(first[1])
+    g.close()
+
+
+def test_config_1024_slice(G, oracle_mod):
+    """The 1024^2 configuration's pipeline on a slice the oracle finishes in
+    seconds: 64 seeded random vortices with zero net charge (Pade ansatz),
+    ARGLE, then RK4 dt = 5e-4, then the energies and all six spectra."""
+    N = 1024
+    say("case 1024^2: ARGLE 10 steps, RK4 4 steps, diagnostics")
+    O = oracle_mod
+    xs, ys, qs = G.random_vortices(64, seed=7)
+    p = G.params_from_namelist(N, 1e-2, 2.5)
+    psi0 = G.ic_vortices_host(N, xs, ys, qs, p.xi, nimg=3)
+    g = G.GPE(p)
+    g.set_psi(psi0, sync=False)
+    st = O.State(O.make_params(N, 1e-2, 2.5))
+    st.set_psi(psi0, sync=False)
+    r, ro = g.ARGLE(nsteps=10), st.argle(10)
+    assert abs(r - ro) <= 1e-9 * abs(ro)
+    kpsi = g.kpsi
+    assert rel_l2(kpsi, st.kpsi) < TOL
+    # real time from the relaxed state, as the reference restarts from ARGLE
+    # output (dealias + psi = IFFT(kpsi))
+    q = G.params_from_namelist(N, 5e-4, 2.5)
+    h = G.GPE(q)
+    h.set_kpsi(np.ascontiguousarray(kpsi))
+    st2 = O.State(O.make_params(N, 5e-4, 2.5))
+    st2.kpsi[...] = kpsi
+    st2.ic_sync()
+    h.gpe_rk4(4)
+    st2.rk4(4)
+    e = rel_l2(h.kpsi, st2.kpsi)
+    d = h.diagnostics_gpe()
+    scal, spec, _ = st2.diagnostics()
+    e_spec = max(rel_l2(d["spec"][i], spec[i]) for i in range(6))
+    say("  rel-L2 kpsi %.2e  worst spectrum %.2e" % (e, e_spec))
+    assert e < TOL and e_spec < TOL
+    for i in (0, 1, 2, 3, 4):
+        assert abs(d["scal"][i] - scal[i]) <= 1e-11 * abs(scal[i])
+    g.close()
+    h.close()
+
+
+def test_conservation_4096():
+    """Full size, where the oracle is too slow: 20 RK4 steps of the 4096^2
+    workload (256 random vortices, dt = 2.5e-4) conserve norm and energy to
+    the scheme's accuracy, and momentum is conserved on the energy scale."""
+    from quantumvortices_amd import gpe as G
+    N = 4096
+    say("case conservation 4096^2")
+    xs, ys, qs = G.random_vortices(256, seed=3)
+    g = G.GPE(G.params_from_namelist(N, 2.5e-4, 2.5))
+    g.set_ic_vortices(xs, ys, qs, nimg=3)
+    d0 = g.diagnostics_gpe()
+    g.gpe_rk4(20)
+    d1 = g.diagnostics_gpe()
+    dn = abs(d1["norm"] - d0["norm"]) / d0["norm"]
+    de = abs(d1["etot"] - d0["etot"]) / abs(d0["etot"])
+    say("  norm %.15e -> %.15e (rel %.2e)  etot %.12e -> %.12e (rel %.2e)  mom %.2e %.2e"
+        % (d0["norm"], d1["norm"], dn, d0["etot"], d1["etot"], de, d1["momx"], d1["momy"]))
+    dm = abs(d1["momx"] - d0["momx"]) + abs(d1["momy"] - d0["momy"])
+    say("  momentum change %.2e" % dm)
+    assert dn < 1e-12 and de < 1e-9
+    assert dm < 1e-9 * (abs(d0["etot"]) + abs(d0["momx"]) + abs(d0["momy"]))
+    g.close()
+
+
 # ppotd=2 (width 5 dx): potential and force evaluated in real space inside
 # the nonlinear pass; ppotd=1 (width 2.5 dx): spectral translation path
 @pytest.mark.parametrize("ppotd", [2.0, 1.0])
