@@ -213,6 +213,28 @@ int vk_fno_features(vk_fno* f, int nv, const double* xv, const double* yv, const
 int vk_fno_tf_guess(vk_fno* f, double omega, double g, double vnorm, double* psi_xy);
 int vk_fno_train_step(vk_fno* f, const float* x, const float* tgt, int nb, float lr, double* loss);
 
+/* Checkpoint files in the reference's formats (host only, no device).
+ *  vk_wf_write: io_fields (src/io_fields.f90:41-50): nsplit files
+ *    <dir>/wf<iopoint>p<r>.dat, r = 1..nsplit, Fortran unformatted
+ *    sequential (4-byte record markers): header (INTEGER iopoint, REAL(8)
+ *    time, REAL(8) dt), then N/nsplit records of N COMPLEX(8), one per row
+ *    y of psi_xy (x fastest).
+ *  vk_wf_read: restart_gpe / restart_from_argle_data
+ *    (src/io_fields.f90:71-224, src/initial_config.f90:630-771): reads the
+ *    nsplit_orig files of checkpoint nrestart into one field (regrouping).
+ *  vk_part_checkpoint: gppart_checkpoint (src/gpe_particle.f90:144-184):
+ *    appends 'iopoint ptime a b' lines to <dir>/gppos_ckp.dat_nrst<nrst>,
+ *    gpvel_ckp..., gpf0ext_ckp...  (pos/vel/fext are (2,np) arrays).
+ *  vk_part_restart: initialize_particles (src/gpe_particle.f90:92-141):
+ *    reads np lines of <dir>/gppos.dat_nrst<nrst>, gpvel..., gpf0ext.... */
+int vk_wf_write(const char* dir, int iopoint, double time, double dt, int N, int nsplit,
+                const double* psi_xy);
+int vk_wf_read(const char* dir, int nrestart, int N, int nsplit_orig, int* iopoint, double* time,
+               double* dt, double* psi_xy);
+int vk_part_checkpoint(const char* dir, int nrst, int iopoint, double ptime, int np, const double* pos,
+                       const double* vel, const double* fext);
+int vk_part_restart(const char* dir, int nrst, int np, double* pos, double* vel, double* fext);
+
 const char* vk_last_error(void);
 const char* vk_version(void);
 

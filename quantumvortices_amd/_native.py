@@ -90,6 +90,12 @@ SIGNATURES = {
                                        ctypes.c_double, fp]),
     "vk_fno_tf_guess": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]),
     "vk_fno_train_step": (ctypes.c_int, [ctypes.c_void_p, fp, fp, ctypes.c_int, ctypes.c_float, dp]),
+    "vk_wf_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, dp]),
+    "vk_wf_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, dp, dp]),
+    "vk_part_checkpoint": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                          dp, dp, dp]),
+    "vk_part_restart": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, dp, dp, dp]),
     "vk_last_error": (ctypes.c_char_p, []),
     "vk_version": (ctypes.c_char_p, []),
 }

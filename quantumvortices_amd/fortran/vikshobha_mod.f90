@@ -6,6 +6,8 @@
 !   gpe_rk4                   <- src/gpe.f90:310
 !   ARGLE(time)               <- src/gpe.f90:5
 !   diagnostics_gpe(spoint,jtime,time) <- src/diagnostics.f90:4
+!   io_fields(iopoint,time)   <- src/io_fields.f90:1 (wf files, IOEVERYPROC layout)
+!   restart_gpe(nrestart)     <- src/io_fields.f90:71 (with nsplit regrouping)
 !   clear_fft                 <- src/mod_fft.f90:433
 ! psi(Nx,Ny) and kpsi(Ny,Nx) are host arrays with the reference's layouts.
 ! The state lives on the device between calls; psi/kpsi are refreshed by
@@ -83,6 +85,21 @@ MODULE vikshobha_c
       REAL(C_DOUBLE), VALUE :: xi
       COMPLEX(C_DOUBLE_COMPLEX) :: psi(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_wf_write(dir, iopoint, time, dt, N, nsplit, psi) BIND(C, NAME='vk_wf_write')
+      IMPORT :: C_INT, C_DOUBLE, C_CHAR, C_DOUBLE_COMPLEX
+      CHARACTER(KIND=C_CHAR) :: dir(*)
+      INTEGER(C_INT), VALUE :: iopoint, N, nsplit
+      REAL(C_DOUBLE), VALUE :: time, dt
+      COMPLEX(C_DOUBLE_COMPLEX) :: psi(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_wf_read(dir, nrestart, N, nsplit_orig, iopoint, time, dt, psi) BIND(C, NAME='vk_wf_read')
+      IMPORT :: C_INT, C_DOUBLE, C_CHAR, C_DOUBLE_COMPLEX
+      CHARACTER(KIND=C_CHAR) :: dir(*)
+      INTEGER(C_INT), VALUE :: nrestart, N, nsplit_orig
+      INTEGER(C_INT) :: iopoint
+      REAL(C_DOUBLE) :: time, dt
+      COMPLEX(C_DOUBLE_COMPLEX) :: psi(*)
+    END FUNCTION
     FUNCTION vk_last_error() BIND(C, NAME='vk_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR) :: vk_last_error
@@ -103,6 +120,12 @@ MODULE vikshobha_gpu
   COMPLEX(C_DOUBLE_COMPLEX), ALLOCATABLE, TARGET :: psi(:,:), kpsi(:,:)
   REAL(C_DOUBLE) :: egrad, echem, eintr, etot, norm, momx, momy, n0, argle_residual
   REAL(C_DOUBLE), ALLOCATABLE :: spec_gp(:,:)
+  ! checkpoint layout: slabs written per file (nsplit_current) and slabs the
+  ! restart files were written with (nsplit_orignal); directories as in the
+  ! reference driver
+  INTEGER :: nsplit_orignal = 1, nsplit_current = 1
+  CHARACTER(LEN=256) :: wf_out_dir = 'data/wf', wf_in_dir = 'restart/wf'
+  REAL(C_DOUBLE) :: timestart = 0.0_C_DOUBLE, dt_read = 0.0_C_DOUBLE
 
 CONTAINS
 
@@ -184,6 +207,25 @@ CONTAINS
     CALL vk_check(vk_gpe_diagnostics(vkh, scal, spec_gp, pscal), 'diagnostics_gpe')
     egrad = scal(1); echem = scal(2); eintr = scal(3); etot = scal(4)
     norm = scal(5); momx = scal(6); momy = scal(7); n0 = scal(8)
+  END SUBROUTINE
+
+  ! io_fields (src/io_fields.f90:41-50): psi to wf<iopoint>p<r>.dat
+  SUBROUTINE io_fields(iopoint, time)
+    INTEGER, INTENT(IN) :: iopoint
+    REAL(C_DOUBLE), INTENT(IN) :: time
+    CALL sync_host()
+    CALL vk_check(vk_wf_write(TRIM(wf_out_dir)//C_NULL_CHAR, INT(iopoint, C_INT), time, params%dt, &
+                              INT(Nx, C_INT), INT(nsplit_current, C_INT), psi), 'io_fields')
+  END SUBROUTINE
+
+  ! restart_gpe (src/io_fields.f90:71-224): psi from the nsplit_orignal wf
+  ! files of checkpoint nrestart, then kpsi = FFT(psi)
+  SUBROUTINE restart_gpe(nrestart)
+    INTEGER, INTENT(IN) :: nrestart
+    INTEGER(C_INT) :: icpoint
+    CALL vk_check(vk_wf_read(TRIM(wf_in_dir)//C_NULL_CHAR, INT(nrestart, C_INT), INT(Nx, C_INT), &
+                             INT(nsplit_orignal, C_INT), icpoint, timestart, dt_read, psi), 'restart_gpe')
+    CALL fft('psi', forward)
   END SUBROUTINE
 
   SUBROUTINE clear_fft()

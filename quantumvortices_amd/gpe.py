@@ -146,6 +146,7 @@ class GPE:
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 2)
         vel = np.zeros((n, 2)) if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 2)
         fext = np.zeros((n, 2)) if fext is None else np.ascontiguousarray(fext, dtype=np.float64).reshape(n, 2)
+        self._fext = fext.copy()
         nat.check(nat.lib().vk_gpe_particles_set(self.h, nat.ptr(pos), nat.ptr(vel), nat.ptr(fext)))
 
     def set_obstacles(self, pos, V0, d):
@@ -180,6 +181,41 @@ class GPE:
         nat.check(nat.lib().vk_gpe_particles_get(self.h, nat.ptr(pos), nat.ptr(vel), nat.ptr(frc)))
         return pos, vel, frc
 
+    # ---- checkpoints (the reference's file formats) ------------------------------
+    def io_fields(self, directory, iopoint, time, nsplit=1):
+        """io_fields (src/io_fields.f90:1-62): psi to wf<iopoint>p<r>.dat,
+        r = 1..nsplit, in <directory>."""
+        write_wf(directory, iopoint, time, self.params.dt, self.psi, nsplit)
+
+    def restart_gpe(self, directory, nrestart, nsplit_orig=1, sync=False):
+        """restart_gpe (src/io_fields.f90:71-224): read checkpoint nrestart
+        (nsplit_orig files) and set psi (kpsi = FFT(psi); sync also dealiases
+        and re-synchronises psi, as restart_from_argle_data + ic_otoc_gpe).
+        Returns (iopoint, time, dt) from the file header."""
+        icp, time, dt, psi = read_wf(directory, nrestart, self.N, nsplit_orig)
+        self.set_psi(psi, sync=sync)
+        return icp, time, dt
+
+    def gppart_checkpoint(self, directory, nrst, iopoint, ptime):
+        """gppart_checkpoint (src/gpe_particle.f90:144-184): append positions,
+        velocities and external forces to the *_ckp.dat_nrst<nrst> files."""
+        n = self.params.np
+        pos, vel, _ = self.particles()
+        fext = np.ascontiguousarray(self._fext if getattr(self, "_fext", None) is not None else np.zeros((n, 2)))
+        nat.check(nat.lib().vk_part_checkpoint(str(directory).encode(), int(nrst), int(iopoint), float(ptime), n,
+                                               nat.ptr(np.ascontiguousarray(pos)), nat.ptr(np.ascontiguousarray(vel)),
+                                               nat.ptr(fext)))
+
+    def initialize_particles_from(self, directory, nrst):
+        """initialize_particles (src/gpe_particle.f90:92-141) from
+        gppos/gpvel/gpf0ext.dat_nrst<nrst> in <directory>."""
+        n = self.params.np
+        pos, vel, fext = np.zeros((n, 2)), np.zeros((n, 2)), np.zeros((n, 2))
+        nat.check(nat.lib().vk_part_restart(str(directory).encode(), int(nrst), n, nat.ptr(pos), nat.ptr(vel),
+                                            nat.ptr(fext)))
+        self.initialize_particles(pos, vel, fext)
+        return pos, vel, fext
+
     # ---- timing ----------------------------------------------------------------
     def rk4_async(self, nsteps):
         nat.check(nat.lib().vk_gpe_rk4_steps_async(self.h, int(nsteps)))
@@ -199,6 +235,25 @@ class GPE:
         k = nat.lib().vk_pass_kinds()
         names = [nat.lib().vk_pass_name(i).decode() for i in range(k)]
         return {names[i]: nat.lib().vk_gpe_pass_bytes(self.h, i) for i in range(k)}
+
+
+def write_wf(directory, iopoint, time, dt, psi, nsplit=1):
+    """wf checkpoint files of io_fields (src/io_fields.f90:41-50) from a
+    host psi[y, x] complex128 array."""
+    a = np.ascontiguousarray(psi, dtype=np.complex128)
+    nat.check(nat.lib().vk_wf_write(str(directory).encode(), int(iopoint), float(time), float(dt), a.shape[0],
+                                    int(nsplit), nat.ptr(a)))
+
+
+def read_wf(directory, nrestart, N, nsplit_orig=1):
+    """(iopoint, time, dt, psi[y, x]) from the nsplit_orig wf files of
+    checkpoint nrestart (restart_gpe, src/io_fields.f90:71-224)."""
+    psi = np.empty((N, N), dtype=np.complex128)
+    icp = nat.ctypes.c_int()
+    t, dt = np.zeros(1), np.zeros(1)
+    nat.check(nat.lib().vk_wf_read(str(directory).encode(), int(nrestart), int(N), int(nsplit_orig),
+                                   nat.ctypes.byref(icp), nat.ptr(t), nat.ptr(dt), nat.ptr(psi)))
+    return icp.value, float(t[0]), float(dt[0]), psi
 
 
 class Ensemble:
