@@ -114,6 +114,13 @@ int vk_gpe_set_obstacles(vk_gpe* h, int np, const double* pos, double V0, double
 /* Replaces the time step (dt_fix) of a live handle. */
 int vk_gpe_set_dt(vk_gpe* h, double dt);
 
+/* Advective ARGLE (src/gpe.f90:78-81,91-94,114-146): taylor_green = 1 is
+ * the GPE_ADVECTION build (u = (sin x cos y, -cos x sin y), terms
+ * -i u.grad psi - |u|^2 psi/(4 alpha)); (vnx, vny) != 0 the GPE_COUNTERFLOW
+ * build with namelist gpcounterflow (term -i v.grad psi).  Both may be on;
+ * RK4 is unaffected, as in the reference. */
+int vk_gpe_set_advection(vk_gpe* h, int taylor_green, double vnx, double vny);
+
 /* Conformance hook mirroring fft(plan_name, direction)
  * (src/mod_fft.f90:471-542): dir = -1: in = x-space (Nx,Ny) -> out = k-space
  * (Ny,Nx) transposed, unnormalised; dir = +1: inverse times 1/(Nx*Ny). */

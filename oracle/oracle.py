@@ -41,6 +41,9 @@ class OrcParams(ctypes.Structure):
         ("d", ctypes.c_double),
         ("mass", ctypes.c_double),
         ("force_dealias", ctypes.c_int),
+        ("adv", ctypes.c_int),
+        ("vnx", ctypes.c_double),
+        ("vny", ctypes.c_double),
     ]
 
 
@@ -91,7 +94,7 @@ def set_threads(n):
 
 
 def make_params(N, dt, xi_in=2.5, rho_avg=1.0, disp=0.0, csound=None,
-                np_=0, V0_in=0.0, d_in=0.0, mass=1.0, force_dealias=1):
+                np_=0, V0_in=0.0, d_in=0.0, mass=1.0, force_dealias=1, adv=0, vnx=0.0, vny=0.0):
     """Grid + constants as the reference derives them (src/mod_grid.f90:63-101,
     src/driver.f90:360-393).  Default: g=1 with xi = xi_in*dx, i.e.
     csound = sqrt(2)*xi so that alpha = xi^2."""
@@ -106,6 +109,9 @@ def make_params(N, dt, xi_in=2.5, rho_avg=1.0, disp=0.0, csound=None,
     p.d = d_in * p.xi         # ppotd = ppotd*xi
     p.mass = mass
     p.force_dealias = force_dealias
+    p.adv = adv
+    p.vnx = vnx
+    p.vny = vny
     return p
 
 

@@ -515,7 +515,8 @@ void orc_rk4_step(orc_state* s)
     }
 }
 
-/* src/gpe.f90:5-224 (ARGLE, plain: no advection, counterflow or SGLE).
+/* src/gpe.f90:5-224 (ARGLE without SGLE; with the advective terms of
+ * GPE_ADVECTION and GPE_COUNTERFLOW when p->adv / p->vnx, p->vny are set).
  * residual (not defined by the reference, which has no convergence test):
  *   ||kpsi_new - kpsi_old||_2 / (dt * ||kpsi_old||_2) over all modes. */
 void orc_argle_step(orc_state* s, double* residual)
@@ -524,7 +525,18 @@ void orc_argle_step(orc_state* s, double* residual)
     const int N = p->N;
     size_t n2 = (size_t)N * N;
     const double dt = p->dt;
+    const int cflow = p->vnx != 0.0 || p->vny != 0.0;
     orc_fft2d(N, s->kpsi, s->psi, +1);
+    if (p->adv || cflow) {
+        /* gradients of psi: IFFT(i kx kpsi), IFFT(i ky kpsi) (:78-81,91-94) */
+        FOR_K(s) {
+            size_t k = KIDX(s);
+            s->tmp1[k] = cmul(mkc(0.0, (double)K1), s->kpsi[k]);
+            s->tmp2[k] = cmul(mkc(0.0, (double)K2), s->kpsi[k]);
+        }
+        orc_fft2d(N, s->tmp1, s->tmp1, +1);
+        orc_fft2d(N, s->tmp2, s->tmp2, +1);
+    }
     if (p->np > 0) orc_translate_all(s);
     for (size_t i = 0; i < n2; ++i) s->nlin[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
     orc_fft2d(N, s->nlin, s->nlin, -1);
@@ -537,6 +549,20 @@ void orc_argle_step(orc_state* s, double* residual)
         if (p->np > 0) {
             orc_c pot = cmul(s->trans_v[i], s->psi[i]);
             v = cadd(v, mkc(-RE(pot), -IM(pot)));
+        }
+        if (p->adv) {
+            /* Taylor-Green velocity (:114-130):
+             * -i (u.grad psi) - (1/4)(1/alpha)|u|^2 psi */
+            const double x = (double)(i % N) * p->dx, y = (double)(i / N) * p->dx;
+            const double ux = sin(x) * cos(y), uy = -cos(x) * sin(y);
+            const orc_c a = cadd(rmul(ux, s->tmp1[i]), rmul(uy, s->tmp2[i]));
+            v = csub(v, cmul(mkc(0.0, 1.0), a));
+            v = csub(v, rmul(0.25 * (1.0 / p->alpha) * (ux * ux + uy * uy), s->psi[i]));
+        }
+        if (cflow) {
+            /* constant counterflow (:136-146): -i (v.grad psi) */
+            const orc_c a = cadd(rmul(p->vnx, s->tmp1[i]), rmul(p->vny, s->tmp2[i]));
+            v = csub(v, cmul(mkc(0.0, 1.0), a));
         }
         s->nlin[i] = v;
     }

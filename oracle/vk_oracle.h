@@ -37,6 +37,11 @@ typedef struct {
     int np;
     double V0, d, mass;
     int force_dealias;  /* 1: dealias rho-hat in the force with k1^2+k2^2 */
+    /* ARGLE advection (src/gpe.f90:78-81,91-94,114-146): adv = 1 adds the
+     * Taylor-Green term (GPE_ADVECTION); (vnx, vny) != 0 the counterflow
+     * term (GPE_COUNTERFLOW, namelist gpcounterflow) */
+    int adv;
+    double vnx, vny;
 } orc_params;
 
 typedef struct orc_state orc_state;

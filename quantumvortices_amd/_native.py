@@ -53,6 +53,7 @@ SIGNATURES = {
     "vk_gpe_particles_set": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
     "vk_gpe_particles_get": (ctypes.c_int, [ctypes.c_void_p, dp, dp, dp]),
     "vk_gpe_set_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "vk_gpe_set_advection": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "vk_gpe_set_obstacles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]),
     "vk_fft2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
     "vk_ic_vortices_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_double, ctypes.c_int, dp]),

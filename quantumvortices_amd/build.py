@@ -41,15 +41,33 @@ def build_fortran(verbose=False):
 
 
 def build(verbose=False):
+    """Compile every source to an object in parallel (each translation unit
+    holds its own kernels), then link the shared library."""
+    import tempfile
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRC
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + err)
+    cflags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory() as td:
+        jobs = []
+        for src in SRC:
+            obj = os.path.join(td, os.path.basename(src) + ".o")
+            cmd = [HIPCC] + cflags + ["-c", "-o", obj, src]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            jobs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        errs = []
+        for obj, pr in jobs:
+            _, e = pr.communicate()
+            if pr.returncode != 0:
+                errs.append("\n".join(l for l in e.splitlines() if "packed-fp32-ops" not in l))
+        if errs:
+            raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in jobs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
     if os.path.exists(FORTRAN):
         build_fortran(verbose)

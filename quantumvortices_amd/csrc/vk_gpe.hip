@@ -88,6 +88,10 @@ struct vk_gpe {
     cplx *K0 = nullptr, *K1 = nullptr, *K2 = nullptr, *Hpsi = nullptr, *Hrho = nullptr, *HN = nullptr;
     cplx* Xs = nullptr;   // x-space psi rows written by P1 and read by P3 (xpsi)
     int xpsi = 0;
+    // advective ARGLE (Taylor-Green and/or counterflow); Hdy = IFFT_y(i ky kpsi)
+    int adv_tg = 0;
+    double adv_vx = 0.0, adv_vy = 0.0;
+    cplx* Hdy = nullptr;
     cplx *HV = nullptr, *KV = nullptr, *Krho = nullptr, *Vhat = nullptr, *EX = nullptr, *EY = nullptr;
     double *fpart = nullptr, *pst = nullptr, *part = nullptr, *dsum = nullptr;
     int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
@@ -470,6 +474,9 @@ int argle_step(vk_gpe* h, int want_res)
             particle_kv(h, C::TC * h->geo.Pp);
             h->kv_valid = 1;
         }
+        const bool adv = h->adv_tg || h->adv_vx != 0.0 || h->adv_vy != 0.0;
+        if (adv) col_launch<C>(h, PK_OTHER, h->NP, vk::k_col_inv<C>, (const cplx*)h->K0, h->Hdy, (const cplx*)h->tw,
+                               h->geo, 0, 2);   // i ky kpsi -> Hdy (all columns)
         if (h->xpsi)
             row_launch<C>(h, PK_P1, vk::k_p1x<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc,
                           h->Xs);
@@ -477,7 +484,17 @@ int argle_step(vk_gpe* h, int want_res)
             row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc);
         col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                       (const cplx*)h->tw, h->geo, (parts && !rs) ? 1 : 0);
-        if (h->xpsi) {
+        if (adv) {
+            const vk::AdvArgs aa{h->adv_tg, h->adv_vx, h->adv_vy, 0.25 / h->p.alpha, sc};
+            const cplx* xs = h->xpsi ? (const cplx*)h->Xs : nullptr;
+            auto launch = [&](auto k3) {
+                row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV,
+                                (const cplx*)h->Hpsi, xs, (const cplx*)h->Hdy, h->HN, (const cplx*)h->tw, h->geo,
+                                -h->p.g * sc, -sc, sc, -1.0, pa, aa);
+            };
+            if (h->xpsi) launch(!parts ? vk::k_p3a<C, 0, true> : (rs ? vk::k_p3a<C, 2, true> : vk::k_p3a<C, 1, true>));
+            else launch(!parts ? vk::k_p3a<C, 0, false> : (rs ? vk::k_p3a<C, 2, false> : vk::k_p3a<C, 1, false>));
+        } else if (h->xpsi) {
             auto k3 = !parts ? vk::k_p3x<C, 0> : (rs ? vk::k_p3x<C, 2> : vk::k_p3x<C, 1>);
             row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Xs,
                             h->HN, (const cplx*)h->tw, h->geo, 0, -h->p.g * sc, -sc, -1.0, pa);
@@ -742,7 +759,7 @@ void vk_gpe_destroy(vk_gpe* h)
 {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
-    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd, h->Xs};
+    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd, h->Xs, h->Hdy};
     for (auto f : fields) if (f) hipFree(f);
     if (h->np_cap > 0) {
         cplx* pf[] = {h->HV, h->KV, h->Krho, h->Vhat, h->EX, h->EY};
@@ -925,6 +942,21 @@ int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel, const 
     h->kv_valid = 0;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int vk_gpe_set_advection(vk_gpe* h, int taylor_green, double vnx, double vny)
+{
+    if (!h || (taylor_green != 0 && taylor_green != 1) || !std::isfinite(vnx) || !std::isfinite(vny))
+        return fail(VK_ERR_ARG, "set_advection: taylor_green must be 0 or 1 and the velocity finite");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if ((taylor_green || vnx != 0.0 || vny != 0.0) && !h->Hdy) {
+        HIPCHK(hipMalloc(&h->Hdy, f2(h) * sizeof(cplx)));
+        HIPCHK(hipMemset(h->Hdy, 0, f2(h) * sizeof(cplx)));
+    }
+    h->adv_tg = taylor_green;
+    h->adv_vx = vnx;
+    h->adv_vy = vny;
     return 0;
 }
 

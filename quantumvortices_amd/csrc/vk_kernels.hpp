@@ -350,6 +350,15 @@ __device__ __forceinline__ void pcoord(double a, double& X, double& dX)
     dX = (1.0 + 0.5 * s2 + 0.125 * s2 * s2) * cs;
 }
 
+// Advective ARGLE terms (src/gpe.f90:78-81,91-94,114-146): tg adds the
+// Taylor-Green velocity u = (sin x cos y, -cos x sin y) as
+// -i u.grad psi - |u|^2 psi / (4 alpha); (vx, vy) the constant counterflow
+// -i v.grad psi.  c4 = 1/(4 alpha); sc normalises the gradient transforms.
+struct AdvArgs {
+    int tg;
+    double vx, vy, c4, sc;
+};
+
 // P3 (row): rho_d (+ i V) row from the kx >= 0 halves (Hermitian), psi row
 // from Hpsi, N = (gs*rho_d + V) * psi, forward along x -> HN.
 // gs = +-g/N^2, vs = +-1/N^2 (RK4 +, ARGLE -); psi scaled by 1/N^2.
@@ -358,12 +367,15 @@ __device__ __forceinline__ void pcoord(double a, double& X, double& dX)
 // vsign: +1 RK4, -1 ARGLE.
 // XPSI: psi comes from the x-space rows P1 stored (already scaled), so the
 // factor f stays in registers and no transform of Hpsi runs here.
-template <class C, int PM, bool XPSI = false>
+// ADV: advective ARGLE; grad psi from the rows of i kx Hpsi and of Hdy =
+// IFFT_y(i ky kpsi), two more transforms.
+template <class C, int PM, bool XPSI = false, bool ADV = false>
 __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
                                                 const cplx* __restrict__ Hpsi, cplx* __restrict__ HN,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
                                                 int store_pmode, double gs, double vs, double scale,
-                                                double vsign, P3Part pa, const cplx* __restrict__ Xpsi = nullptr)
+                                                double vsign, P3Part pa, const cplx* __restrict__ Xpsi = nullptr,
+                                                const cplx* __restrict__ Hdy = nullptr, AdvArgs aa = AdvArgs{})
 {
     constexpr int flags = PM;
     VK_ROW_SETUP(C)
@@ -492,6 +504,18 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
             }
         }
     }
+    // Taylor-Green u at (x, y) from the twiddle table: tw[k] = e^{-2 pi i k/N}
+    const double cy_ = tw[y].x, sy_ = -tw[y].y;
+    if constexpr (ADV) {
+        if (aa.tg) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const cplx e = tw[t + T * s];
+                const double ux = -e.y * cy_, uy = -e.x * sy_;
+                VK_F(s) -= aa.c4 * (ux * ux + uy * uy);
+            }
+        }
+    }
     if constexpr (XPSI) {
         const cplx* xr = Xpsi + (size_t)y * N;
 #pragma unroll
@@ -510,6 +534,36 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
         }
     }
 #undef VK_F
+    if constexpr (ADV) {
+        // -i (u + v).grad psi, one gradient component at a time
+        cplx w[S];
+#pragma unroll 1
+        for (int comp = 0; comp < 2; ++comp) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int pp = VK_PANEL(s);
+                cplx a = czero();
+                if (panel_stored(pp, NP, g, psi_pmode)) {
+                    if (comp == 0) {
+                        a = Hpsi[hoff<C>(pp, y, c)];
+                        const double kk = (double)ksg(t + T * s, N);
+                        a = mk(-kk * a.y, kk * a.x);   // i kx
+                    } else {
+                        a = Hdy[hoff<C>(pp, y, c)];
+                    }
+                }
+                w[s] = conjc(a);
+            }
+            fft_fwd<N, S, C::SPLIT>(w, lg, t, tw);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const cplx e = tw[t + T * s];
+                const double u = comp == 0 ? (aa.tg ? -e.y * cy_ : 0.0) + aa.vx : (aa.tg ? -e.x * sy_ : 0.0) + aa.vy;
+                const cplx d = rmul(aa.sc, conjc(w[s]));   // d psi / dx_comp
+                v[s] = cadd(v[s], mk(u * d.y, -u * d.x));  // -i u d
+            }
+        }
+    }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -526,6 +580,17 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3(const cplx* __restrict__ 
                                                 double vsign, P3Part pa)
 {
     p3_body<C, PM>(blockIdx.x, gridDim.x, Hrhod, HV, Hpsi, HN, tw, g, psi_pmode, store_pmode, gs, vs, scale, vsign, pa);
+}
+
+template <class C, int PM, bool XPSI>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3a(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+                                                 const cplx* __restrict__ Hpsi, const cplx* __restrict__ Xpsi,
+                                                 const cplx* __restrict__ Hdy, cplx* __restrict__ HN,
+                                                 const cplx* __restrict__ tw, Geo g, double gs, double vs,
+                                                 double scale, double vsign, P3Part pa, AdvArgs aa)
+{
+    p3_body<C, PM, XPSI, true>(blockIdx.x, gridDim.x, Hrhod, HV, Hpsi, HN, tw, g, 0, 0, gs, vs, XPSI ? 1.0 : scale,
+                               vsign, pa, Xpsi, Hdy, aa);
 }
 
 template <class C, int PM>

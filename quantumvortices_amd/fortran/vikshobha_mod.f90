@@ -100,6 +100,12 @@ MODULE vikshobha_c
       REAL(C_DOUBLE) :: time, dt
       COMPLEX(C_DOUBLE_COMPLEX) :: psi(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_set_advection(h, tg, vnx, vny) BIND(C, NAME='vk_gpe_set_advection')
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: tg
+      REAL(C_DOUBLE), VALUE :: vnx, vny
+    END FUNCTION
     FUNCTION vk_last_error() BIND(C, NAME='vk_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR) :: vk_last_error
@@ -226,6 +232,17 @@ CONTAINS
     CALL vk_check(vk_wf_read(TRIM(wf_in_dir)//C_NULL_CHAR, INT(nrestart, C_INT), INT(Nx, C_INT), &
                              INT(nsplit_orignal, C_INT), icpoint, timestart, dt_read, psi), 'restart_gpe')
     CALL fft('psi', forward)
+  END SUBROUTINE
+
+  ! the GPE_ADVECTION / GPE_COUNTERFLOW builds of ARGLE (src/gpe.f90:78-146):
+  ! taylor_green selects the Taylor-Green term, (vnx, vny) the counterflow
+  SUBROUTINE set_argle_advection(taylor_green, vnx, vny)
+    LOGICAL, INTENT(IN) :: taylor_green
+    REAL(C_DOUBLE), INTENT(IN) :: vnx, vny
+    INTEGER(C_INT) :: tg
+    tg = 0
+    IF (taylor_green) tg = 1
+    CALL vk_check(vk_gpe_set_advection(vkh, tg, vnx, vny), 'set_argle_advection')
   END SUBROUTINE
 
   SUBROUTINE clear_fft()

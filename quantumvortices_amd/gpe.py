@@ -160,6 +160,11 @@ class GPE:
         self.params.V0 = float(V0)
         self.params.d = float(d)
 
+    def set_advection(self, taylor_green=False, vnx=0.0, vny=0.0):
+        """ARGLE's advective terms: the Taylor-Green velocity (GPE_ADVECTION)
+        and/or a constant counterflow (GPE_COUNTERFLOW, vnx, vny)."""
+        nat.check(nat.lib().vk_gpe_set_advection(self.h, 1 if taylor_green else 0, float(vnx), float(vny)))
+
     def set_dt(self, dt):
         """Replace dt_fix of the live state."""
         nat.check(nat.lib().vk_gpe_set_dt(self.h, float(dt)))

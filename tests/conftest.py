@@ -12,6 +12,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X)")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """When GPU tests run, initialise torch's ROCm runtime before the
+    library's: torch carries its own HIP runtime, and it finds no device if
+    another HIP runtime in the process already opened the GPU."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+        torch.cuda.is_available()
+    yield
+
+
 @pytest.fixture(scope="session")
 def oracle_mod():
     import oracle as O

@@ -183,6 +183,45 @@ def test_diagnostics_vs_oracle(G, oracle_mod, N):
     g.close()
 
 
+@pytest.mark.parametrize("N,tg,vx,vy,nobs,steps", [(64, 1, 0.0, 0.0, 0, 10), (128, 0, 0.7, -0.4, 0, 10),
+                                                     (128, 1, 0.3, 0.2, 0, 10), (64, 1, 0.5, 0.0, 2, 8),
+                                                     (2048, 1, 0.3, -0.2, 0, 3)])
+def test_advective_argle_vs_oracle(G, oracle_mod, N, tg, vx, vy, nobs, steps):
+    """ARGLE with the Taylor-Green advection and/or counterflow terms
+    (GPE_ADVECTION / GPE_COUNTERFLOW builds of src/gpe.f90:5-224)."""
+    say("case advective argle N=%d tg=%d v=(%g,%g) obstacles=%d" % (N, tg, vx, vy, nobs))
+    O = oracle_mod
+    dt = 1e-2
+    psi0 = _ic(G, N)
+    kw, okw = {}, {}
+    if nobs:
+        kw = dict(gp_N_obj=nobs, ppotV0=10.0, ppotd=2.0, gp_mass_part=1.0)
+        okw = dict(np_=nobs, V0_in=10.0, d_in=2.0, mass=1.0)
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5, **kw))
+    g.set_advection(bool(tg), vx, vy)
+    g.set_psi(psi0, sync=False)
+    st = O.State(O.make_params(N, dt, 2.5, adv=tg, vnx=vx, vny=vy, **okw))
+    st.set_psi(psi0, sync=False)
+    if nobs:
+        pos = np.array([[1.1, 2.3], [4.4, 0.7]])[:nobs]
+        g.initialize_particles(pos)
+        st.part(0)[...] = pos
+        st.particle_potential()
+    r, ro = g.ARGLE(nsteps=steps), st.argle(steps)
+    e = rel_l2(g.kpsi, st.kpsi)
+    say("  rel-L2 kpsi %.3e residual %.6e / %.6e" % (e, r, ro))
+    assert e < TOL and abs(r - ro) <= 1e-9 * abs(ro)
+    # the advection changes the result: not the plain update
+    st0 = O.State(O.make_params(N, dt, 2.5, **okw))
+    st0.set_psi(psi0, sync=False)
+    if nobs:
+        st0.part(0)[...] = pos
+        st0.particle_potential()
+    st0.argle(steps)
+    assert rel_l2(st0.kpsi, st.kpsi) > 1e-6
+    g.close()
+
+
 def test_reference_cpu_case_128(G, oracle_mod):
     """The reference's own CPU-runnable configuration end to end: 128^2,
     g = 1, xi = 2.5 dx, 4x4 lattice of alternating vortices (Pade cores),

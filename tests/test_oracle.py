@@ -282,6 +282,67 @@ def test_argle_linear_closed_form(oracle_mod):
     assert abs(r - r_ref) < 1e-12 * r_ref
 
 
+def _adv_state(O, N, dt, adv=0, vnx=0.0, vny=0.0):
+    p = O.make_params(N, dt, 2.5, adv=adv, vnx=vnx, vny=vny)
+    p.g = 0.0
+    return O.State(p), p
+
+
+def test_argle_counterflow_closed_form(oracle_mod):
+    """-i v.grad psi on a plane wave e^{i(m x + n y)} is (vnx m + vny n) psi:
+    with g = 0 every mode keeps its own factor
+    (1 + dt (omega + vnx kx + vny ky)) / (1 + alpha k^2 dt)."""
+    O = oracle_mod
+    N, dt, vnx, vny = 32, 5e-2, 0.7, -0.4
+    st, p = _adv_state(O, N, dt, vnx=vnx, vny=vny)
+    rng = np.random.default_rng(9)
+    K0 = (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) * N
+    st.kpsi[...] = K0
+    st.argle(2)
+    k = kwn_np(N).astype(float)
+    f = (1 + dt * (p.omega + vnx * k[:, None] + vny * k[None, :])) / (1 + p.alpha * _kk(N) * dt)
+    assert rel_l2(st.kpsi, K0 * f ** 2) < 1e-13
+
+
+def test_argle_taylor_green_uniform(oracle_mod):
+    """On psi = 1 the Taylor-Green term is -|u|^2/(4 alpha) with
+    |u|^2 = (1 - cos 2x cos 2y)/2: one step moves only the (0,0) and
+    (+-2,+-2) modes, by exact amounts."""
+    O = oracle_mod
+    N, dt = 32, 1e-2
+    st, p = _adv_state(O, N, dt, adv=1)
+    st.set_psi(np.ones((N, N), complex), sync=False)
+    st.argle(1)
+    K = st.kpsi
+    a, n2 = p.alpha, N * N
+    exp = np.zeros((N, N), complex)
+    exp[0, 0] = n2 + dt * (p.omega * n2 - n2 / (8 * a))
+    for kx in (2, -2):
+        for ky in (2, -2):
+            exp[kx % N, ky % N] = dt * n2 / (32 * a) / (1 + 8 * a * dt)
+    assert np.abs(K - exp).max() < 1e-10 * n2
+
+
+def test_argle_taylor_green_plane_wave(oracle_mod):
+    """psi = e^{ix}: u.grad psi with the analytic gradient i e^{ix} x-hat
+    (the oracle takes it spectrally); fixes the sign of -i, the velocity
+    components and the 1/(4 alpha)."""
+    O = oracle_mod
+    N, dt = 32, 1e-2
+    st, p = _adv_state(O, N, dt, adv=1)
+    x = p.dx * np.arange(N)
+    X, Y = x[None, :], x[:, None]
+    psi = np.exp(1j * X) * np.ones((N, 1))
+    st.set_psi(psi, sync=False)
+    st.argle(1)
+    ux, uy = np.sin(X) * np.cos(Y), -np.cos(X) * np.sin(Y)
+    nl = -1j * (ux * (1j * psi)) - 0.25 / p.alpha * (ux ** 2 + uy ** 2) * psi   # [y, x]
+    NL = np.fft.fft2(nl).T                                                      # [kx, ky]
+    P = np.fft.fft2(psi).T
+    exp = (P + dt * (NL + p.omega * P)) / (1 + p.alpha * _kk(N) * dt)
+    assert rel_l2(st.kpsi, exp) < 1e-13
+
+
 def test_argle_uniform_fixed_point(oracle_mod):
     O = oracle_mod
     st, p = _state(O, 32, 1e-2)
