@@ -16,7 +16,8 @@ plain-C oracle (same algorithm as the reference, 16 unfused 2D FFTs per step
 plus the per-particle transforms) timed on a bounded sample of the same
 workload on the host cores; the batched ensemble; and ARGLE time-to-1e-10
 at 512^2 with the learned warm start against plain ARGLE from the Pade
-ansatz (and against the Thomas-Fermi start).
+ansatz (and against the Thomas-Fermi start); and the spatio-temporal
+spectra at the reference's CompSTS size (128^2, 2048 snapshots).
 """
 import argparse
 import json
@@ -219,6 +220,76 @@ def argle_bench(G, torch, n_cfg, plain_cap):
     }
 
 
+def sts_bench(G, torch):
+    """Spatio-temporal spectra at the size of the reference's CompSTS driver
+    (128^2, Nt = 2048 snapshots): one snapshot of the retained modes per
+    record (a gather on the library's stream), then the windowed time
+    transforms of all retained modes, the shell sums and the kx = 0 lines,
+    with the copies of both spectra to the host."""
+    N, nt = 128, 2048
+    p = G.params_from_namelist(N, 1e-3, 2.5)
+    g = G.GPE(p)
+    xs, ys, qs = G.lattice_vortices(4)
+    g.set_psi(G.ic_vortices_host(N, xs + 0.01, ys + 0.02, qs, p.xi), sync=True)
+    sts = G.STS(g, nt)
+    ext = torch.cuda.ExternalStream(g.stream)
+    for _ in range(nt):                       # untimed: first launches
+        sts.record()
+    sts.compute()
+    sts.reset()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(ext)
+    for _ in range(nt):
+        sts.record()
+    e1.record(ext)
+    sts.compute()
+    e2.record(ext)
+    torch.cuda.synchronize()
+    rec_ms, comp_ms = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    kt = sts.kernel_times(20)
+    nm = sts.nmodes
+    # time transforms: read the snapshots (16 B per mode and snapshot), write
+    # the power (8 B); shell sums: read the power once
+    bytes_time = 24.0 * nm * nt
+    bytes_bin = 8.0 * nm * nt
+    sts.close()
+    g.close()
+    ach = bytes_time / (kt[0] * 1e-3) / 1e9
+    return {"workload": "%d^2, Nt=%d snapshots, %d retained modes, %d shells" % (N, nt, nm, sts.nshells),
+            "record_us_per_snapshot": round(1e3 * rec_ms / nt, 3),
+            "compute_ms": round(comp_ms, 4),
+            "kernel_ms": {"k_sts_time": round(kt[0], 4), "k_sts_bin": round(kt[1], 4), "k_sts_x0": round(kt[2], 4)},
+            "roofline": {"bound": "hbm", "kernel": "k_sts_time", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_launch": bytes_time,
+                         "avg_ms": round(kt[0], 4)},
+            "shell_sum_GBps": round(bytes_bin / (kt[1] * 1e-3) / 1e9, 1)}
+
+
+def cpu_baseline_sts(G, threads):
+    """CPU leg of the spectra: the numpy oracle (CompSTS restated: Hann
+    window, numpy's FFT along time, nint(|k|) shells) on the 128^2 x 2048
+    snapshots of the same run, fetched from the device."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import sts_oracle
+    N, nt = 128, 2048
+    p = G.params_from_namelist(N, 1e-3, 2.5)
+    g = G.GPE(p)
+    xs, ys, qs = G.lattice_vortices(4)
+    g.set_psi(G.ic_vortices_host(N, xs + 0.01, ys + 0.02, qs, p.xi), sync=True)
+    sts = G.STS(g, nt)
+    for _ in range(nt):
+        sts.record()
+    snaps = np.array([sts.snapshot(n) for n in range(nt)])
+    sts.close()
+    g.close()
+    t0 = time.perf_counter()
+    sts_oracle.sts_spectra(snaps)
+    sec = time.perf_counter() - t0
+    return {"value": round(sec * 1e3, 1), "unit": "ms per spectra computation", "cores": 1, "kind": "port",
+            "sample": "the full 128^2 x 2048 computation (numpy FFT, single thread)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +304,7 @@ def main():
     ap.add_argument("--no-argle", action="store_true")
     ap.add_argument("--argle-configs", type=int, default=2)
     ap.add_argument("--argle-plain-cap", type=int, default=400000)
+    ap.add_argument("--no-sts", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the single-field path shards as "
@@ -293,12 +365,21 @@ def main():
         argle = argle_bench(G, torch, args.argle_configs, args.argle_plain_cap)
         log("bench: argle", argle)
 
+    sts = None
+    if not args.no_sts:
+        log("bench: spatio-temporal spectra 128^2 x 2048")
+        sts = sts_bench(G, torch)
+        log("bench: sts", sts)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("bench: CPU baseline (oracle) with %d threads" % threads)
         cpu = cpu_baseline(G, N, dt, wl, threads, psi_for_cpu)
         log("bench: cpu", cpu)
+        if sts is not None:
+            sts["cpu_baseline"] = cpu_baseline_sts(G, threads)
+            log("bench: sts cpu", sts["cpu_baseline"])
 
     out = {
         "metric": METRIC,
@@ -323,6 +404,7 @@ def main():
         "cpu_baseline": cpu,
         "ensemble": ens,
         "argle_warm_start": argle,
+        "spatio_temporal_spectra": sts,
     }
     print(json.dumps(out), flush=True)
 

@@ -220,6 +220,34 @@ int vk_fno_features(vk_fno* f, int nv, const double* xv, const double* yv, const
 int vk_fno_tf_guess(vk_fno* f, double omega, double g, double vnorm, double* psi_xy);
 int vk_fno_train_step(vk_fno* f, const float* x, const float* tgt, int nb, float lr, double* loss);
 
+/* Spatio-temporal spectra (sts_gpe, src/diagnostics.f90:636-695, with the
+ * post-processing of Run/CompSTS/driver.f90:134-265), resident on the
+ * device: vk_sts_record snapshots the handle's kpsi (retained modes) after
+ * the caller's steps; once nt snapshots are taken, vk_sts_compute applies
+ * the Hann window (1 - cos(2 pi n/(nt-1)))/2, the forward DFT in time and
+ * stsnorm |.|^2, stsnorm = (1/nt)^2 (1/N^2)^2, and returns
+ *   spec2d[w][sh], sh = nint(|k|) in [0, vk_sts_shells()), summed per shell;
+ *   spec1d[w][ky], the kx = 0 modes.
+ * nt is a power of two in [32, 4096].  vk_sts_get_x0 returns the recorded
+ * kpsi(ky, kx=0) columns (count x N complex), the content of the
+ * reference's kpsi_x0 file. */
+typedef struct vk_sts vk_sts;
+int vk_sts_create(vk_sts** s, vk_gpe* h, int nt);
+void vk_sts_destroy(vk_sts* s);
+int vk_sts_record(vk_sts* s);
+int vk_sts_reset(vk_sts* s);
+int vk_sts_count(const vk_sts* s);
+int vk_sts_modes(const vk_sts* s);
+int vk_sts_shells(const vk_sts* s);
+int vk_sts_compute(vk_sts* s, double* spec2d, double* spec1d);
+int vk_sts_get_x0(vk_sts* s, double* x0);
+/* timing support: average duration (ms) of the compute kernels (time
+ * transforms, shell sums, kx = 0 lines) over reps launches each */
+int vk_sts_kernel_times(vk_sts* s, int reps, double* ms3);
+/* snapshot n (0-based) as a full kpsi_t plane, zero off the retained modes:
+ * the records of the reference's kpsi_nrst<n>p<r>.dat files. */
+int vk_sts_get_kpsi(vk_sts* s, int n, double* kpsi_t);
+
 /* Checkpoint files in the reference's formats (host only, no device).
  *  vk_wf_write: io_fields (src/io_fields.f90:41-50): nsplit files
  *    <dir>/wf<iopoint>p<r>.dat, r = 1..nsplit, Fortran unformatted

@@ -91,6 +91,17 @@ SIGNATURES = {
                                        ctypes.c_double, fp]),
     "vk_fno_tf_guess": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]),
     "vk_fno_train_step": (ctypes.c_int, [ctypes.c_void_p, fp, fp, ctypes.c_int, ctypes.c_float, dp]),
+    "vk_sts_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_int]),
+    "vk_sts_destroy": (None, [ctypes.c_void_p]),
+    "vk_sts_record": (ctypes.c_int, [ctypes.c_void_p]),
+    "vk_sts_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "vk_sts_count": (ctypes.c_int, [ctypes.c_void_p]),
+    "vk_sts_modes": (ctypes.c_int, [ctypes.c_void_p]),
+    "vk_sts_shells": (ctypes.c_int, [ctypes.c_void_p]),
+    "vk_sts_compute": (ctypes.c_int, [ctypes.c_void_p, dp, dp]),
+    "vk_sts_get_x0": (ctypes.c_int, [ctypes.c_void_p, dp]),
+    "vk_sts_get_kpsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
+    "vk_sts_kernel_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
     "vk_wf_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                    ctypes.c_int, dp]),
     "vk_wf_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, dp, dp]),

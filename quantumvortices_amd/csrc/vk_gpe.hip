@@ -945,6 +945,15 @@ int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel, const 
     return 0;
 }
 
+}  // extern "C"
+
+// accessors for the library's other translation units (not part of the C ABI)
+cplx* vk_internal_kpsi(vk_gpe* h) { return h ? h->K0 : nullptr; }
+int vk_internal_N(const vk_gpe* h) { return h ? h->N : 0; }
+hipStream_t vk_internal_stream(const vk_gpe* h) { return h ? h->stream : nullptr; }
+
+extern "C" {
+
 int vk_gpe_set_advection(vk_gpe* h, int taylor_green, double vnx, double vny)
 {
     if (!h || (taylor_green != 0 && taylor_green != 1) || !std::isfinite(vnx) || !std::isfinite(vny))

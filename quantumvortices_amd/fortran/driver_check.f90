@@ -1,7 +1,9 @@
 ! A reference-style driver on the ISO_C_BINDING layer: sets up a 4x4 Padé
 ! vortex lattice at N=64, syncs it like the reference's restart path, runs
 ! RK4 steps and diagnostics through the reference's subroutine names, and
-! writes psi, kpsi and the energies to a stream file for comparison.
+! writes psi, kpsi and the energies to a stream file for comparison; then
+! takes Ntt = 32 spatio-temporal snapshots, one per RK4 step, and appends
+! spec_sts2d and arr1.
 PROGRAM driver_check
   USE ISO_C_BINDING
   USE vikshobha_c
@@ -38,6 +40,13 @@ PROGRAM driver_check
   CALL GET_COMMAND_ARGUMENT(1, outname)
   OPEN(NEWUNIT=u, FILE=TRIM(outname), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
   WRITE(u) psi, kpsi, egrad, echem, eintr, etot, norm, momx, momy, spec_gp
+  Ntt = 32
+  DO i = 1, Ntt
+    CALL gpe_rk4()
+    CALL sts_gpe(i, (nsteps + i) * 1.0e-3_C_DOUBLE)
+  END DO
+  CALL comp_sts()
+  WRITE(u) spec_sts2d, arr1
   CLOSE(u)
   WRITE(*,'(A,ES22.14,A,ES22.14)') 'driver_check: norm=', norm, ' etot=', etot
   CALL clear_fft()

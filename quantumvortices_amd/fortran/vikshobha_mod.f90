@@ -106,6 +106,29 @@ MODULE vikshobha_c
       INTEGER(C_INT), VALUE :: tg
       REAL(C_DOUBLE), VALUE :: vnx, vny
     END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_sts_create(s, h, nt) BIND(C, NAME='vk_sts_create')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR) :: s
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: nt
+    END FUNCTION
+    SUBROUTINE vk_sts_destroy(s) BIND(C, NAME='vk_sts_destroy')
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: s
+    END SUBROUTINE
+    INTEGER(C_INT) FUNCTION vk_sts_record(s) BIND(C, NAME='vk_sts_record')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: s
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_sts_shells(s) BIND(C, NAME='vk_sts_shells')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: s
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_sts_compute(s, spec2d, spec1d) BIND(C, NAME='vk_sts_compute')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: s
+      REAL(C_DOUBLE) :: spec2d(*), spec1d(*)
+    END FUNCTION
     FUNCTION vk_last_error() BIND(C, NAME='vk_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR) :: vk_last_error
@@ -132,6 +155,11 @@ MODULE vikshobha_gpu
   INTEGER :: nsplit_orignal = 1, nsplit_current = 1
   CHARACTER(LEN=256) :: wf_out_dir = 'data/wf', wf_in_dir = 'restart/wf'
   REAL(C_DOUBLE) :: timestart = 0.0_C_DOUBLE, dt_read = 0.0_C_DOUBLE
+  ! spatio-temporal spectra: Ntt snapshots (the CompSTS driver's Nt = Ntt),
+  ! spec_sts2d(0:shells-1, 1:Ntt) and arr1(1:Ny, 1:Ntt) as CompSTS computes them
+  INTEGER :: Ntt = 2048
+  TYPE(C_PTR) :: vks = C_NULL_PTR
+  REAL(C_DOUBLE), ALLOCATABLE :: spec_sts2d(:,:), arr1(:,:)
 
 CONTAINS
 
@@ -245,7 +273,28 @@ CONTAINS
     CALL vk_check(vk_gpe_set_advection(vkh, tg, vnx, vny), 'set_argle_advection')
   END SUBROUTINE
 
+  ! sts_gpe (src/diagnostics.f90:636-695): one snapshot of kpsi, kept on the
+  ! device (the first call allocates Ntt snapshots)
+  SUBROUTINE sts_gpe(stspoint, time)
+    INTEGER, INTENT(IN) :: stspoint
+    REAL(C_DOUBLE), INTENT(IN) :: time
+    IF (.NOT. C_ASSOCIATED(vks)) THEN
+      CALL vk_check(vk_sts_create(vks, vkh, INT(Ntt, C_INT)), 'sts_gpe')
+      ALLOCATE(spec_sts2d(0:vk_sts_shells(vks) - 1, 1:Ntt), arr1(1:Ny, 1:Ntt))
+    END IF
+    CALL vk_check(vk_sts_record(vks), 'sts_gpe')
+  END SUBROUTINE
+
+  ! the CompSTS post-processing (Run/CompSTS/driver.f90:134-265) once Ntt
+  ! snapshots are taken: fills spec_sts2d and arr1
+  SUBROUTINE comp_sts()
+    CALL vk_check(vk_sts_compute(vks, spec_sts2d, arr1), 'comp_sts')
+  END SUBROUTINE
+
   SUBROUTINE clear_fft()
+    IF (C_ASSOCIATED(vks)) CALL vk_sts_destroy(vks)
+    vks = C_NULL_PTR
+    IF (ALLOCATED(spec_sts2d)) DEALLOCATE(spec_sts2d, arr1)
     IF (C_ASSOCIATED(vkh)) CALL vk_gpe_destroy(vkh)
     vkh = C_NULL_PTR
     IF (ALLOCATED(psi)) DEALLOCATE(psi, kpsi, spec_gp)

@@ -242,6 +242,133 @@ class GPE:
         return {names[i]: nat.lib().vk_gpe_pass_bytes(self.h, i) for i in range(k)}
 
 
+def _fortran_e(x, w=15, d=6):
+    """Fortran Ew.d edit descriptor (0.dddddddE+xx, right-justified)."""
+    if x == 0.0:
+        m, e = 0.0, 0
+    else:
+        e = int(math.floor(math.log10(abs(x)))) + 1
+        m = abs(x) / 10.0 ** e
+        if round(m, d) >= 1.0:
+            m, e = m / 10.0, e + 1
+    t = "%s0.%0*dE%+03d" % ("-" if x < 0 else "", d, int(round(m * 10 ** d)), e)
+    return t.rjust(w)
+
+
+def _fortran_record(f, a):
+    b = np.ascontiguousarray(a).tobytes()
+    m = np.array([len(b)], dtype="<i4").tobytes()
+    f.write(m + b + m)
+
+
+class STS:
+    """Spatio-temporal spectra of one GPE state: sts_gpe
+    (src/diagnostics.f90:636-695) snapshots kpsi every sampling interval,
+    and the CompSTS post-processing (Run/CompSTS/driver.f90:134-265) turns
+    nt snapshots into spectra.  Here the snapshots stay on the device:
+    record(time) after each interval, compute() after nt of them returns
+    spec2d[w, shell] (shells 0 .. nshells-1, nint(|k|)) and spec1d[w, ky]
+    (the kx = 0 modes), frequency index w in FFT order."""
+
+    def __init__(self, gpe, nt):
+        h = nat.ctypes.c_void_p()
+        nat.check(nat.lib().vk_sts_create(nat.ctypes.byref(h), gpe.h, int(nt)))
+        self.h, self.gpe, self.nt, self.N = h, gpe, int(nt), gpe.N
+        self.nshells = nat.lib().vk_sts_shells(self.h)
+        self.nmodes = nat.lib().vk_sts_modes(self.h)
+        self.times = []
+
+    def close(self):
+        if getattr(self, "h", None) and nat is not None:
+            nat.lib().vk_sts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sts_gpe(self, time=0.0):
+        """Snapshot the current kpsi (the reference's sts_gpe)."""
+        nat.check(nat.lib().vk_sts_record(self.h))
+        self.times.append(float(time))
+
+    record = sts_gpe
+
+    def reset(self):
+        nat.check(nat.lib().vk_sts_reset(self.h))
+        self.times = []
+
+    @property
+    def count(self):
+        return nat.lib().vk_sts_count(self.h)
+
+    def compute(self):
+        s2 = np.empty((self.nt, self.nshells))
+        s1 = np.empty((self.nt, self.N))
+        nat.check(nat.lib().vk_sts_compute(self.h, nat.ptr(s2), nat.ptr(s1)))
+        return s2, s1
+
+    def kernel_times(self, reps=10):
+        """Average ms of the compute kernels: (time transforms, shell sums, kx=0 lines)."""
+        out = np.zeros(3)
+        nat.check(nat.lib().vk_sts_kernel_times(self.h, int(reps), nat.ptr(out)))
+        return tuple(out)
+
+    def x0(self):
+        """Recorded kpsi(ky, kx=0) columns, (count, N) complex128."""
+        out = np.empty((self.count, self.N), dtype=np.complex128)
+        nat.check(nat.lib().vk_sts_get_x0(self.h, nat.ptr(out)))
+        return out
+
+    def snapshot(self, n):
+        """Recorded snapshot n as kpsi_t[kx, ky] (zero off the retained modes)."""
+        out = np.empty((self.N, self.N), dtype=np.complex128)
+        nat.check(nat.lib().vk_sts_get_kpsi(self.h, int(n), nat.ptr(out)))
+        return out
+
+    def write_sts_files(self, directory, nrst, nsplit=1):
+        """The files sts_gpe appends during a run, written at once (see
+        write_sts_files)."""
+        write_sts_files(directory, nrst, self.times, self.x0(),
+                        (self.snapshot(n) for n in range(self.count)), self.N, nsplit)
+
+    def write_spectra(self, directory, nrst, spec2d, spec1d):
+        """CompSTS's text outputs: sts1d_kx0_nrst<n>.dat (one line of Ny
+        values per frequency) and sts2d_nrst<n>.dat (shells 0 .. int(Nx/3))."""
+        import os
+        np.savetxt(os.path.join(directory, "sts1d_kx0_nrst%d.dat" % nrst), spec1d, fmt="%25.17E")
+        np.savetxt(os.path.join(directory, "sts2d_nrst%d.dat" % nrst), spec2d[:, :self.N // 3 + 1], fmt="%25.17E")
+
+
+def write_sts_files(directory, nrst, times, x0, planes, N, nsplit=1):
+    """The files of sts_gpe (src/diagnostics.f90:636-695): sts_time.dat_nrst<n>
+    (one E15.6 line per snapshot), sts1d/kpsi_x0_nrst<n>.dat (one unformatted
+    record of Ny complex per snapshot) and sts2d/kpsi_nrst<n>p<r>.dat (per
+    slab of N/nsplit kx columns, one record of Ny x N/nsplit complex per
+    snapshot, ky fastest).  planes yields kpsi_t[kx, ky] per snapshot."""
+    import os
+    if N % nsplit:
+        raise ValueError("nsplit must divide N")
+    os.makedirs(os.path.join(directory, "sts1d"), exist_ok=True)
+    os.makedirs(os.path.join(directory, "sts2d"), exist_ok=True)
+    with open(os.path.join(directory, "sts_time.dat_nrst%d" % nrst), "w") as f:
+        for t in times:
+            f.write(_fortran_e(t) + "\n")
+    with open(os.path.join(directory, "sts1d", "kpsi_x0_nrst%d.dat" % nrst), "wb") as f:
+        for row in x0:
+            _fortran_record(f, np.asarray(row, dtype=np.complex128))
+    L = N // nsplit
+    files = [open(os.path.join(directory, "sts2d", "kpsi_nrst%dp%d.dat" % (nrst, r + 1)), "wb")
+             for r in range(nsplit)]
+    try:
+        for k in planes:
+            k = np.asarray(k, dtype=np.complex128)
+            for r in range(nsplit):
+                _fortran_record(files[r], k[r * L:(r + 1) * L])
+    finally:
+        for f in files:
+            f.close()
+
+
 def write_wf(directory, iopoint, time, dt, psi, nsplit=1):
     """wf checkpoint files of io_fields (src/io_fields.f90:41-50) from a
     host psi[y, x] complex128 array."""

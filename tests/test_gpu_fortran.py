@@ -26,7 +26,9 @@ def test_fortran_driver(tmp_path, oracle_mod):
     psi = np.frombuffer(raw[:N * N * 16], dtype=np.complex128).reshape(N, N)
     kpsi = np.frombuffer(raw[N * N * 16:2 * N * N * 16], dtype=np.complex128).reshape(N, N)
     sc = np.frombuffer(raw[2 * N * N * 16:2 * N * N * 16 + 7 * 8], dtype=np.float64)
-    spec = np.frombuffer(raw[2 * N * N * 16 + 7 * 8:], dtype=np.float64).reshape(6, ns + 1)
+    o = 2 * N * N * 16 + 7 * 8
+    spec = np.frombuffer(raw[o:o + 6 * (ns + 1) * 8], dtype=np.float64).reshape(6, ns + 1)
+    o += 6 * (ns + 1) * 8
     from quantumvortices_amd import gpe as G
     p = G.params_from_namelist(N, 1e-3, 2.5)
     assert G.nshell(N) == ns
@@ -46,4 +48,15 @@ def test_fortran_driver(tmp_path, oracle_mod):
     assert rel_l2(psi, st.psi) < 1e-11
     scal, _, _ = st.diagnostics()
     assert abs(sc[4] - scal[4]) < 1e-11 * scal[4] and abs(sc[3] - scal[3]) < 1e-11 * abs(scal[3])
+    # sts_gpe + comp_sts: 32 snapshots, one per step
+    nt = 32
+    sts = G.STS(g, nt)
+    for _ in range(nt):
+        g.gpe_rk4(1)
+        sts.record()
+    s2, s1 = sts.compute()
+    f2 = np.frombuffer(raw[o:o + nt * sts.nshells * 8], dtype=np.float64).reshape(nt, sts.nshells)
+    f1 = np.frombuffer(raw[o + nt * sts.nshells * 8:], dtype=np.float64).reshape(nt, N)
+    assert np.array_equal(f2, s2) and np.array_equal(f1, s1)
+    sts.close()
     g.close()
