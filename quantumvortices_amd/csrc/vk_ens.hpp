@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(256) k_ens_p4(const EnsReal* __restrict__ R, c
     a.wacc = (stage == 1 || stage == 4) ? e.dt * oney6 : e.dt * oney6 * 2.0;
     a.cstep = stage == 3 ? e.dt : e.dt * 0.5;
     a.r1 = (e.dt * oney6) / (e.dt * 0.5);
+    a.r3 = (e.dt * oney6 * 2.0) / e.dt;
     VK_ENS_SWITCH(e.N, (p4_body<C>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a)))
 }
 

@@ -454,6 +454,7 @@ int rk4_step(vk_gpe* h)
             a.wacc = (stage == 1 || stage == 4) ? dt * oney6 : dt * oney6 * 2.0;
             a.cstep = stage == 3 ? dt : dt * 0.5;
             a.r1 = (dt * oney6) / (dt * 0.5);
+            a.r3 = (dt * oney6 * 2.0) / dt;
             col_launch<C>(h, PK_P4, h->geo.Pp + h->geo.Pn, vk::k_p4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
                           h->K2, (const cplx*)h->tw, h->geo, a);
         }
@@ -1127,7 +1128,7 @@ double vk_gpe_pass_bytes(vk_gpe* h, int kind)
     case PK_P1: return F * (fr + fp + (h->xpsi ? 1.0 : 0.0));   // + psi rows written for P3
     case PK_P2: return F * 2.0 * fp;
     case PK_P3: return F * (fp + (h->xpsi ? 1.0 : fr) + fr);    // psi rows instead of Hpsi
-    case PK_P4: return F * (fr + fr) + D * (2 + 4 + 5 + 4) / 4.0;   // K traffic per stage
+    case PK_P4: return F * (fr + fr) + D * (2 + 4 + 4 + 4) / 4.0;   // K traffic per stage
     default: return 0.0;
     }
 }

@@ -609,6 +609,7 @@ struct RKArgs {
     double wacc;    // dt/6 (stages 1,4) or dt/6*2 (stages 2,3)
     double cstep;   // dt/2, dt/2, dt, - for stages 1..3
     double r1;      // stage 2: str2 after stage 1 = r1 (K2 - K0), r1 = (dt/6)/(dt/2)
+    double r3;      // stage 4: str2 after stage 3 = K1 + r3 (K2 - K0), r3 = (dt/3)/dt
     int stage;
 };
 
@@ -616,11 +617,12 @@ struct RKArgs {
 // RHS = pref*((alpha k^2 - omega) psi_s-hat + N-hat) on the retained disk;
 // RK4 accumulation and stage update (src/gpe.f90:366-487);
 // inverse along y of the new kpsi -> Hpsi.
-// K0 = str1 (= kpsi at step start), K1 = str2, K2 = stage kpsi.  Stage 1
-// does not store str2: it equals (dt/6) rhs1 = r1 (K2 - K0), which stage 2
-// rebuilds from the two arrays it reads anyway (one fewer write and read
-// of a field per step; the difference is one rounding of K2, ~1 ulp of
-// kpsi).
+// K0 = str1 (= kpsi at step start), K1 = str2, K2 = stage kpsi.  Stages 1
+// and 3 do not store str2: the term they would add, (dt/6) rhs1 or
+// (dt/3) rhs3, equals r (K2 - K0) with the K2 they write, and the next
+// stage rebuilds it from arrays it reads anyway (two fewer field writes
+// and one fewer field read per step; each costs one rounding of K2, ~1 ulp
+// of kpsi).
 template <class C>
 __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
@@ -647,9 +649,12 @@ __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restr
             const cplx lin = rmul(a.alpha * kr2 - a.omega, ps);
             const cplx rhs = cmul(a.pref, cadd(lin, v[s]));
             const cplx inc = rmul(a.wacc, rhs);
-            const cplx acc = a.stage == 1 ? inc : a.stage == 2 ? cadd(rmul(a.r1, csub(ps, s1)), inc) : cadd(K1[o], inc);
+            cplx acc = inc;
+            if (a.stage == 2) acc = cadd(rmul(a.r1, csub(ps, s1)), inc);
+            else if (a.stage == 3) acc = cadd(K1[o], inc);
+            else if (a.stage == 4) acc = cadd(cadd(K1[o], rmul(a.r3, csub(ps, s1))), inc);
             if (a.stage < 4) {
-                if (a.stage > 1) K1[o] = acc;
+                if (a.stage == 2) K1[o] = acc;
                 kn = cadd(s1, rmul(a.cstep, rhs));
                 K2[o] = kn;
             } else {
