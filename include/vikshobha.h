@@ -230,7 +230,8 @@ int vk_fno_train_step(vk_fno* f, const float* x, const float* tgt, int nb, float
  *   spec1d[w][ky], the kx = 0 modes.
  * nt is a power of two in [32, 4096].  vk_sts_get_x0 returns the recorded
  * kpsi(ky, kx=0) columns (count x N complex), the content of the
- * reference's kpsi_x0 file. */
+ * reference's kpsi_x0 file.  An sts object uses its handle's stream and
+ * kpsi: destroy it before the handle. */
 typedef struct vk_sts vk_sts;
 int vk_sts_create(vk_sts** s, vk_gpe* h, int nt);
 void vk_sts_destroy(vk_sts* s);
