@@ -94,6 +94,7 @@ struct vk_gpe {
     cplx* Hdy = nullptr;
     cplx *HV = nullptr, *KV = nullptr, *Krho = nullptr, *Vhat = nullptr, *EX = nullptr, *EY = nullptr;
     double *fpart = nullptr, *pst = nullptr, *part = nullptr, *dsum = nullptr;
+    unsigned char* phit = nullptr;   // real-space path: window flags per row block and particle
     int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
     int kv_valid = 0;
     int have_part = 0;
@@ -201,8 +202,7 @@ template <class C> size_t p3_extra(const vk_gpe* h)
 {
     const bool park = !h->xpsi || (h->have_part && h->p.np > 0 && h->rs_part);
     const size_t np = (size_t)std::max(h->p.np, 1);
-    return (park ? (size_t)C::S * C::ROWT * sizeof(double) : 0) + 2 * np * sizeof(double) +
-           (np + 31) / 32 * sizeof(unsigned);
+    return (park ? (size_t)C::S * C::ROWT * sizeof(double) : 0) + 2 * np * sizeof(double);
 }
 
 template <class C, class Kern, class... A>
@@ -333,6 +333,7 @@ vk::P3Part p3part(const vk_gpe* h)
         const double pi = 4.0 * std::atan(1.0);
         q.pos = h->pst;
         q.ppart = h->fpart;
+        q.phit = h->phit;
         q.np = h->p.np;
         q.V0 = h->p.V0;
         q.d = h->p.d;
@@ -393,8 +394,10 @@ int particle_step_rs(vk_gpe* h, int nblk, int stage)
     ev_begin(h);
     const double dx = 2.0 * (4.0 * std::atan(1.0)) / h->N;
     const double coef = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
-    hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, st, (const double*)h->fpart, nblk,
-                       h->p.np, coef, h->pst + 28 * h->p.np + 8);
+    int nw = 1;
+    dispatch(h->N, [&](auto n) { nw = (CfgN<decltype(n)::value>::ROWT + 63) / 64; return 0; });
+    hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, st, (const double*)h->fpart,
+                       (const unsigned char*)h->phit, nblk, nw, h->p.np, coef, h->pst + 28 * h->p.np + 8);
     hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, st, (const double*)h->fpart, (const cplx*)h->EX,
                        h->pst, part_args(h, 0, 0, stage, 1, nblk));
     ev_end(h, PK_PART);
@@ -732,9 +735,20 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
             if (hipMalloc(b, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particles)"));
             hipMemset(*b, 0, fb);
         }
+        // force partials: spectral path 4 per row kx and particle; real-space
+        // path 2 per row block, wave and particle
+        size_t rows = 0, nw = 0;
+        dispatch(N, [&](auto n) {
+            using C = CfgN<decltype(n)::value>;
+            rows = C::N / C::TR;
+            nw = (C::ROWT + 63) / 64;
+            return 0;
+        });
+        const size_t nfp = std::max((size_t)4 * np * N, 2 * rows * nw * np);
         if (hipMalloc(&h->EX, sizeof(cplx) * np * N) != hipSuccess ||
             hipMalloc(&h->EY, sizeof(cplx) * np * N) != hipSuccess ||
-            hipMalloc(&h->fpart, sizeof(double) * 4 * np * N) != hipSuccess ||
+            hipMalloc(&h->fpart, sizeof(double) * nfp) != hipSuccess ||
+            hipMalloc(&h->phit, rows * np) != hipSuccess ||
             hipMalloc(&h->pst, sizeof(double) * (30 * np + 8)) != hipSuccess)
             return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particle tables)"));
         hipMemset(h->pst, 0, sizeof(double) * (30 * np + 8));
@@ -767,6 +781,7 @@ void vk_gpe_destroy(vk_gpe* h)
         cplx* pf[] = {h->HV, h->KV, h->Krho, h->Vhat, h->EX, h->EY};
         for (auto f : pf) if (f) hipFree(f);
         if (h->fpart) hipFree(h->fpart);
+        if (h->phit) hipFree(h->phit);
         if (h->pst) hipFree(h->pst);
     }
     for (auto x : h->X) if (x) hipFree(x);

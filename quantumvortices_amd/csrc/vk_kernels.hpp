@@ -327,7 +327,8 @@ __device__ __forceinline__ void block_sum(double (&x)[NV], double* sh)
 // FP64 roundoff; see the host-side selection).
 struct P3Part {
     const double* pos;   // (x, y) per particle
-    double* ppart;       // per row block and particle: sum rho_raw * dV/dx, dV/dy
+    double* ppart;       // per row block, wave and particle: sum rho_raw * dV/dx, dV/dy
+    unsigned char* phit; // per row block and particle: 1 if the window reaches the block
     int np;
     double V0, d, W, dx; // potential height, width, window radius, grid spacing
 };
@@ -386,12 +387,8 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     // those during the first transform, read back as broadcasts
     constexpr bool FREG = XPSI && (flags & 2) == 0;
     double* qlds = lds + TR * C::ROWRS + (FREG ? 0 : S * C::ROWT);
-    // hit mask (one bit per particle whose window reaches this row block)
-    // after the positions
-    unsigned* hmask = reinterpret_cast<unsigned*>(qlds + 2 * pa.np);
     if constexpr ((flags & 2) != 0) {
         for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
-        for (int i = threadIdx.x; i < (pa.np + 31) / 32; i += blockDim.x) hmask[i] = 0u;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -432,74 +429,82 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
         const int yb = blk * TR;
         const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
         const double idx = 1.0 / pa.dx, Wg = pa.W * idx, invN = 1.0 / N;
-        // window test, one thread per particle, in grid units
-        // (multiplications only): minimum-image row distance of the
-        // workgroup's rows to the particle; particles that miss get zero
-        // force partials here
-        for (int pp = threadIdx.x; pp < pa.np; pp += blockDim.x) {
-            const double cy = qlds[2 * pp + 1] * idx;
+        // The particle loop has no workgroup barrier: every wave tests the
+        // windows itself (one lane per particle, a ballot gives the hit set,
+        // the same in every wave), reduces its force partials with lane
+        // shuffles and writes them per wave; the per-block hit flags tell
+        // the reduction which entries were written.
+        constexpr int WV = C::ROWT < 64 ? C::ROWT : 64;   // lanes of a wave in use
+        constexpr int NW = (C::ROWT + 63) / 64;            // waves per workgroup
+        const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+        for (int base = 0; base < pa.np; base += WV) {
+            // window test in grid units (multiplications only): minimum-image
+            // row distance of the workgroup's rows to the particle
+            const int pq = base + lane;
             bool hit = false;
-            for (int r2 = 0; r2 < TR; ++r2) {
-                double dr = (yb + r2) - cy;
+            if (lane < WV && pq < pa.np) {
+                const double cyq = qlds[2 * pq + 1] * idx;
+                for (int r2 = 0; r2 < TR; ++r2) {
+                    double dr = (yb + r2) - cyq;
+                    dr -= N * rint(dr * invN);
+                    hit |= fabs(dr) <= Wg;
+                }
+                if constexpr ((flags & 4) != 0)
+                    if (wv == 0) pa.phit[(size_t)blk * pa.np + pq] = hit ? 1 : 0;
+            }
+            // hit particles in increasing order (uniform over the wave)
+            for (unsigned long long m = __ballot(hit); m != 0ull; m &= m - 1ull) {
+                const int pp = base + __builtin_ctzll(m);
+                const double qx = qlds[2 * pp], qy = qlds[2 * pp + 1];
+                const double cx = qx * idx, cy = qy * idx;   // centre in grid units
+                double fr[2] = {0.0, 0.0};
+                double dr = y - cy;
                 dr -= N * rint(dr * invN);
-                hit |= fabs(dr) <= Wg;
-            }
-            if (hit) {
-                atomicOr(&hmask[pp >> 5], 1u << (pp & 31));
-            } else if constexpr ((flags & 4) != 0) {
-                pa.ppart[((size_t)blk * pa.np + pp) * 2] = 0.0;
-                pa.ppart[((size_t)blk * pa.np + pp) * 2 + 1] = 0.0;
-            }
-        }
-        __syncthreads();
-        // hit particles in increasing order (uniform over the workgroup)
-        for (int w = 0; w < (pa.np + 31) / 32; ++w)
-        for (unsigned m = hmask[w]; m != 0u; m &= m - 1u) {
-            const int pp = 32 * w + __builtin_ctz(m);
-            const double qx = qlds[2 * pp], qy = qlds[2 * pp + 1];
-            const double cx = qx * idx, cy = qy * idx;   // centre in grid units
-            double fr[2] = {0.0, 0.0};
-            double dr = y - cy;
-            dr -= N * rint(dr * invN);
-            if (fabs(dr) <= Wg) {
-                double Y, dY;
-                pcoord(pwrap(y * pa.dx - qy), Y, dY);
-                // points of this thread inside the window (cheap test for
-                // all S), then the transcendental work once per such point,
-                // in increasing s (a single point when the window is
-                // narrower than the thread stride T)
-                unsigned mask = 0;
+                if (fabs(dr) <= Wg) {
+                    double Y, dY;
+                    pcoord(pwrap(y * pa.dx - qy), Y, dY);
+                    // points of this thread inside the window (cheap test for
+                    // all S), then the transcendental work once per such point,
+                    // in increasing s (a single point when the window is
+                    // narrower than the thread stride T)
+                    unsigned mask = 0;
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    double dc = (t + T * s) - cx;
-                    dc -= N * rint(dc * invN);
-                    if (fabs(dc) <= Wg) mask |= 1u << s;
-                }
-                while (mask) {
-                    const int sw = __builtin_ctz(mask);
-                    mask &= mask - 1;
-                    double rs = 0.0;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) rs = (s == sw) ? rho[s] : rs;
-                    double X, dX;
-                    pcoord(pwrap((t + T * sw) * pa.dx - qx), X, dX);
-                    const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
-                    if constexpr (FREG) {
-#pragma unroll
-                        for (int s = 0; s < S; ++s)
-                            if (s == sw) fr_[FREG ? s : 0] += vsign * e;
-                    } else {
-                        fl[sw * C::ROWT + threadIdx.x] += vsign * e;
+                    for (int s = 0; s < S; ++s) {
+                        double dc = (t + T * s) - cx;
+                        dc -= N * rint(dc * invN);
+                        if (fabs(dc) <= Wg) mask |= 1u << s;
                     }
-                    fr[0] += rs * (-e * X * dX * invd2);
-                    fr[1] += rs * (-e * Y * dY * invd2);
+                    while (mask) {
+                        const int sw = __builtin_ctz(mask);
+                        mask &= mask - 1;
+                        double rs = 0.0;
+#pragma unroll
+                        for (int s = 0; s < S; ++s) rs = (s == sw) ? rho[s] : rs;
+                        double X, dX;
+                        pcoord(pwrap((t + T * sw) * pa.dx - qx), X, dX);
+                        const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
+                        if constexpr (FREG) {
+#pragma unroll
+                            for (int s = 0; s < S; ++s)
+                                if (s == sw) fr_[FREG ? s : 0] += vsign * e;
+                        } else {
+                            fl[sw * C::ROWT + threadIdx.x] += vsign * e;
+                        }
+                        fr[0] += rs * (-e * X * dX * invd2);
+                        fr[1] += rs * (-e * Y * dY * invd2);
+                    }
                 }
-            }
-            if constexpr ((flags & 4) != 0) {
-                block_sum<2>(fr, lds);
-                if (threadIdx.x == 0) {
-                    pa.ppart[((size_t)blk * pa.np + pp) * 2] = fr[0];
-                    pa.ppart[((size_t)blk * pa.np + pp) * 2 + 1] = fr[1];
+                if constexpr ((flags & 4) != 0) {
+#pragma unroll
+                    for (int o = WV / 2; o > 0; o >>= 1) {
+                        fr[0] += __shfl_xor(fr[0], o, WV);
+                        fr[1] += __shfl_xor(fr[1], o, WV);
+                    }
+                    if (lane == 0) {
+                        double* q = pa.ppart + (((size_t)blk * NW + wv) * pa.np + pp) * 2;
+                        q[0] = fr[0];
+                        q[1] = fr[1];
+                    }
                 }
             }
         }
@@ -943,17 +948,21 @@ __global__ void k_force_rows(const cplx* __restrict__ Krho, const cplx* __restri
                 for (int k = 0; k < 4; ++k) fpart[((size_t)kx * np + p0 + j) * 4 + k] = acc[4 * j + k];
 }
 
-// fl[p] = coef * sum over row blocks b (fixed order) of ppart[b][p] (x, y);
-// one workgroup per particle
-__global__ void k_pforce_rs(const double* __restrict__ ppart, int nblk, int np, double coef,
-                            double* __restrict__ fl)
+// fl[p] = coef * sum over the row blocks b whose window flag is set, in a
+// fixed order, of the nw wave partials ppart[b][w][p] (x, y); one workgroup
+// per particle
+__global__ void k_pforce_rs(const double* __restrict__ ppart, const unsigned char* __restrict__ phit, int nblk,
+                            int nw, int np, double coef, double* __restrict__ fl)
 {
     __shared__ double sh[2 * 16];
     const int pp = blockIdx.x;
     double acc[2] = {0.0, 0.0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        acc[0] += ppart[((size_t)b * np + pp) * 2];
-        acc[1] += ppart[((size_t)b * np + pp) * 2 + 1];
+        if (!phit[(size_t)b * np + pp]) continue;
+        for (int w = 0; w < nw; ++w) {
+            acc[0] += ppart[(((size_t)b * nw + w) * np + pp) * 2];
+            acc[1] += ppart[(((size_t)b * nw + w) * np + pp) * 2 + 1];
+        }
     }
     block_sum<2>(acc, sh);
     if (threadIdx.x == 0) {
