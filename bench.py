@@ -307,8 +307,8 @@ def main():
     ap.add_argument("--no-sts", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the single-field path shards as "
-              "1D slabs over RCCL in design only); run with --gpus 1", file=sys.stderr)
+        print("bench.py: runs on one GPU only (the single-field path would shard as "
+              "1D slabs over RCCL; that is not implemented); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

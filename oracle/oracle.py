@@ -1,8 +1,8 @@
 """ctypes view of the CPU oracle (oracle/vk_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker (or the timed CPU
-baseline), never as the product path.
+cpu_baseline leg of bench.py, always as the reference results the tests
+compare against (or the timed CPU baseline), never as the product path.
 
 Arrays follow the reference's Fortran views written as C arrays:
 x-space psi[y, x] (x fastest) and k-space kpsi[kx, ky] (ky fastest, the

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the 2D Gross-Pitaevskii pseudospectral step (test
- * infrastructure only: it is the checker for the HIP library, never the
- * product path).
+ * infrastructure only: the tests compare the HIP library against it; it is
+ * never the product path).
  *
  * Every function restates one piece of the reference Fortran, operation by
  * operation, and cites the file:line it follows (paths relative to the
