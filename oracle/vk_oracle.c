@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load this library, and only as the
- * checker (or the timed CPU baseline) -- never as the product path.
+ * reference the tests compare against (or the timed CPU baseline) -- never
+ * as the product path.
  *
  * Each function cites the reference file:line it follows.  The reference
  * uses FFTW3-MPI for its transforms (an external library, not vendored,

@@ -1,5 +1,5 @@
-"""Pins the CPU oracle (oracle/vk_oracle.c) before it is trusted as the
-checker.  The reference ships no tests, fixtures or golden vectors for this
+"""Pins the CPU oracle (oracle/vk_oracle.c) before the GPU tests trust it.
+The reference ships no tests, fixtures or golden vectors for this
 path and its transform library (FFTW3-MPI) is absent, so every oracle
 function is pinned by what the mathematics fixes, each check chosen so that a
 plausible mistake (dropped term, wrong sign, index or scale, transposed
