@@ -215,6 +215,27 @@ void col_launch(vk_gpe* h, int kind, int nblocks, Kern k, A... args)
     ev_end(h, kind);
 }
 
+// P2 columns per workgroup.  P2 covers only the kx >= 0 panels: at 4096^2
+// its 683 two-column workgroups fill 512 slots (2 per CU) 1.33 times, and
+// one column per workgroup (1366 in fuller waves) is 4% faster; at 2048^2
+// the same split is 10% slower (16-byte row pieces of 4-column panels).
+// Results are bitwise the same either way.
+template <class C> constexpr int p2_cols() { return C::N == 4096 ? 1 : C::TC; }
+
+template <class C, class... A>
+void p2_launch(vk_gpe* h, A... args)
+{
+    constexpr int G = p2_cols<C>(), GS = C::TC / G;
+    auto k = vk::k_p2<C, G>;
+    const size_t lds = (size_t)G * C::COLRS * sizeof(double);
+    set_lds(k, lds);
+    const int npan = h->geo.Pp;
+    const int grid = GS == 1 ? npan : (npan + 7) / 8 * 8 * GS;
+    ev_begin(h);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(G * C::T), lds, h->stream, args...);
+    ev_end(h, PK_P2);
+}
+
 int ncols(const vk_gpe* h, int pmode)
 {
     if (pmode == 0) return h->NP;
@@ -439,7 +460,7 @@ int rk4_step(vk_gpe* h)
                               h->Xs);
             else
                 row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc);
-            col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
+            p2_launch<C>(h, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                           (const cplx*)h->tw, h->geo, (parts && !rs) ? 3 : 0);
             if (parts && !rs) particle_forces(h, h->Krho, Kp, 1, stage);
             if (rs) join_particles(h);   // positions from the previous stage
@@ -487,7 +508,7 @@ int argle_step(vk_gpe* h, int want_res)
                           h->Xs);
         else
             row_launch<C>(h, PK_P1, vk::k_p1<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc);
-        col_launch<C>(h, PK_P2, h->geo.Pp, vk::k_p2<C>, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
+        p2_launch<C>(h, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                       (const cplx*)h->tw, h->geo, (parts && !rs) ? 1 : 0);
         if (adv) {
             const vk::AdvArgs aa{h->adv_tg, h->adv_vx, h->adv_vy, 0.25 / h->p.alpha, sc};

@@ -51,6 +51,28 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     const int kx = p * TC + c;                                             \
     double* lg = lds + c * C::COLRS;
 
+// Column-pass setup for G columns per workgroup (G divides TC).  For
+// G < TC the TC/G workgroups of a panel are siblings placed on one XCD:
+// blocks come in groups of 8*TC/G covering 8 panels, and blocks b and b + 8
+// (consecutive on one XCD under round-robin dispatch) share a panel.  Blocks
+// past NPAN panels return here, before any barrier.
+#define VK_COL_SETUP_G(C, G, B, NPAN)                                       \
+    constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC;                \
+    constexpr int GS_ = TC / (G);                                          \
+    extern __shared__ double lds[];                                        \
+    int p = (B), hh_ = 0;                                                  \
+    if constexpr (GS_ > 1) {                                               \
+        const int r_ = p % (8 * GS_);                                      \
+        hh_ = r_ / 8;                                                      \
+        p = p / (8 * GS_) * 8 + r_ % 8;                                    \
+        if (p >= (NPAN)) return;                                           \
+    }                                                                      \
+    const int u = threadIdx.x;                                             \
+    const int cl_ = u % (G), t = u / (G);                                  \
+    const int c = hh_ * (G) + cl_;                                         \
+    const int kx = p * TC + c;                                             \
+    double* lg = lds + cl_ * C::COLRS;
+
 // panel holding kx = t + T*s in a row pass
 #define VK_PANEL(s) (pidx0 + (T / TC) * (s))
 
@@ -234,12 +256,12 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1x(const cplx* __restrict__
 // (P_G), inverse along y, in place.  flags & 1: also the particle
 // potential column HV = IFFT_y(KV); flags & 2: store the dealiased
 // rho-hat row into K-space Krho (for the particle force).
-template <class C>
+template <class C, int G = C::TC>
 __device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
-    VK_COL_SETUP(C, blk_)
+    VK_COL_SETUP_G(C, G, blk_, g.Pp)
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = Hrho[hoff<C>(p, t + T * s, c)];
@@ -267,12 +289,12 @@ __device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ 
     }
 }
 
-template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
+template <class C, int G = C::TC>
+__global__ void __launch_bounds__(G * C::T) VK_OCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
-    p2_body<C>(blockIdx.x, gridDim.x, Hrho, HV, KV, Krho, tw, g, flags);
+    p2_body<C, G>(blockIdx.x, gridDim.x, Hrho, HV, KV, Krho, tw, g, flags);
 }
 
 // deterministic workgroup sum of NV values per thread; result valid in
