@@ -723,8 +723,8 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
     const size_t fb = f2(h) * sizeof(cplx);
     cplx** bufs[] = {&h->K0, &h->K1, &h->K2, &h->Hpsi, &h->Hrho, &h->HN};
     for (auto b : bufs) {
-        if (hipMalloc(b, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (fields)"));
-        hipMemset(*b, 0, fb);
+        if (hipMalloc(b, fb) != hipSuccess || hipMemset(*b, 0, fb) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (fields)"));
     }
     h->xpsi = N >= VK_XPSI_MIN_N;
     if (h->xpsi && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
@@ -753,8 +753,8 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         const int np = p->np;
         cplx** pb[] = {&h->HV, &h->KV, &h->Krho, &h->Vhat};
         for (auto b : pb) {
-            if (hipMalloc(b, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particles)"));
-            hipMemset(*b, 0, fb);
+            if (hipMalloc(b, fb) != hipSuccess || hipMemset(*b, 0, fb) != hipSuccess)
+                return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particles)"));
         }
         // force partials: spectral path 4 per row kx and particle; real-space
         // path 2 per row block, wave and particle
@@ -772,7 +772,8 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
             hipMalloc(&h->phit, rows * np) != hipSuccess ||
             hipMalloc(&h->pst, sizeof(double) * (30 * np + 8)) != hipSuccess)
             return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (particle tables)"));
-        hipMemset(h->pst, 0, sizeof(double) * (30 * np + 8));
+        if (hipMemset(h->pst, 0, sizeof(double) * (30 * np + 8)) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "hipMemset failed (particle state)"));
         if (hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_p3, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_ps, hipEventDisableTiming) != hipSuccess)
