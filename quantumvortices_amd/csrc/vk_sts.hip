@@ -37,6 +37,17 @@ namespace {
         if (e_ != hipSuccess) return vk_set_error(VK_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Modes per time-transform workgroup.  Fewer modes than the column passes'
+// TC give more, smaller workgroups per CU whose load and transform phases
+// overlap (each mode's series is contiguous, so no access gets narrower):
+// measured at 128^2 x 2048 0.077 -> 0.056 ms with 2 modes, at 64^2 x 4096
+// 0.046 -> 0.032 ms with 1, at 256^2 x 512 0.068 -> 0.050 ms with 2.  Short
+// series keep TC so that a workgroup still fills a wave.  Also the LDS
+// stride between the transform regions (staggered so that interleaved
+// transforms start on different banks).
+template <class C> constexpr int sts_g() { return C::N >= 4096 ? 1 : (C::N >= 256 ? 2 : C::TC); }
+template <class C, int G> constexpr int sts_rs() { return C::LDSW + VK_STAGGER / G; }
+
 // X[m][n] = K[kidx[m]] for snapshot n (each mode's time series contiguous)
 __global__ void k_sts_gather(const cplx* __restrict__ K, const int* __restrict__ kidx, int nm, int Nt, int n,
                              cplx* __restrict__ X)
@@ -49,19 +60,19 @@ __global__ void k_sts_gather(const cplx* __restrict__ K, const int* __restrict__
 // contiguous per mode).  P[m][w] = stsnorm |sum_n h_n X[m][n] e^{-2 pi i n w/Nt}|^2
 // with the window h from a table.  Blocks past the last mode return before
 // the transform's barriers (whole workgroups only).
-template <class C>
-__global__ void __launch_bounds__(C::COLT) k_sts_time(const cplx* __restrict__ X, int nm, double* __restrict__ P,
-                                                     const cplx* __restrict__ tw, const double* __restrict__ hann,
-                                                     double stsnorm)
+template <class C, int G>
+__global__ void __launch_bounds__(G * C::T) k_sts_time(const cplx* __restrict__ X, int nm, double* __restrict__ P,
+                                                      const cplx* __restrict__ tw, const double* __restrict__ hann,
+                                                      double stsnorm)
 {
-    constexpr int Nt = C::N, S = C::S, T = C::T, TC = C::TC;
+    constexpr int Nt = C::N, S = C::S, T = C::T;
     extern __shared__ double lds[];
-    if ((int)blockIdx.x * TC >= nm) return;
+    if ((int)blockIdx.x * G >= nm) return;
     const int u = threadIdx.x;
-    const int c = u % TC, t = u / TC;
-    const int m = blockIdx.x * TC + c;
+    const int c = u % G, t = u / G;
+    const int m = blockIdx.x * G + c;
     const bool live = m < nm;
-    double* lg = lds + c * C::COLRS;
+    double* lg = lds + c * sts_rs<C, G>();
     const cplx* xr = X + (size_t)(live ? m : 0) * Nt;
     cplx v[S];
 #pragma unroll
@@ -160,10 +171,12 @@ int launch_compute(vk_sts* s, hipStream_t st, hipEvent_t* ev)
     if (ev) SCHK(hipEventRecord(ev[0], st));
     int rc = dispatch_nt(s->nt, [&](auto n) {
         using C = vk::Cfg<decltype(n)::value>;
-        const size_t lds = (size_t)C::TC * C::COLRS * sizeof(double);
+        constexpr int G = sts_g<C>();
+        const size_t lds = (size_t)G * sts_rs<C, G>() * sizeof(double);
+        auto kern = k_sts_time<C, G>;
         if (lds > 65536)
-            SCHK(hipFuncSetAttribute((const void*)k_sts_time<C>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_sts_time<C>, dim3(nblk(s->nm, C::TC)), dim3(C::COLT), lds, st, (const cplx*)s->X, s->nm,
+            SCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(nblk(s->nm, G)), dim3(G * C::T), lds, st, (const cplx*)s->X, s->nm,
                            s->P, (const cplx*)s->tw, (const double*)s->hann, stsnorm);
         return 0;
     });
