@@ -222,18 +222,24 @@ void col_launch(vk_gpe* h, int kind, int nblocks, Kern k, A... args)
 // Results are bitwise the same either way.
 template <class C> constexpr int p2_cols() { return C::N == 4096 ? 1 : C::TC; }
 
-template <class C, class... A>
-void p2_launch(vk_gpe* h, A... args)
+// column pass with G columns per workgroup over npan panels
+template <class C, int G, class Kern, class... A>
+void colg_launch(vk_gpe* h, int kind, int npan, Kern k, A... args)
 {
-    constexpr int G = p2_cols<C>(), GS = C::TC / G;
-    auto k = vk::k_p2<C, G>;
+    constexpr int GS = C::TC / G;
     const size_t lds = (size_t)G * C::COLRS * sizeof(double);
     set_lds(k, lds);
-    const int npan = h->geo.Pp;
     const int grid = GS == 1 ? npan : (npan + 7) / 8 * 8 * GS;
     ev_begin(h);
     hipLaunchKernelGGL(k, dim3(grid), dim3(G * C::T), lds, h->stream, args...);
-    ev_end(h, PK_P2);
+    ev_end(h, kind);
+}
+
+template <class C, class... A>
+void p2_launch(vk_gpe* h, A... args)
+{
+    constexpr int G = p2_cols<C>();
+    colg_launch<C, G>(h, PK_P2, h->geo.Pp, vk::k_p2<C, G>, args...);
 }
 
 int ncols(const vk_gpe* h, int pmode)

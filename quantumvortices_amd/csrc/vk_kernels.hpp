@@ -56,17 +56,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
 // blocks come in groups of 8*TC/G covering 8 panels, and blocks b and b + 8
 // (consecutive on one XCD under round-robin dispatch) share a panel.  Blocks
 // past NPAN panels return here, before any barrier.
-#define VK_COL_SETUP_G(C, G, B, NPAN)                                       \
+#define VK_COL_SETUP_G(C, G, B, PEXPR, NPAN)                                \
     constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC;                \
+    constexpr int NP = C::NP;                                              \
+    (void)NP;                                                              \
     constexpr int GS_ = TC / (G);                                          \
     extern __shared__ double lds[];                                        \
-    int p = (B), hh_ = 0;                                                  \
+    int pb_ = (B), hh_ = 0;                                                \
     if constexpr (GS_ > 1) {                                               \
-        const int r_ = p % (8 * GS_);                                      \
+        const int r_ = pb_ % (8 * GS_);                                    \
         hh_ = r_ / 8;                                                      \
-        p = p / (8 * GS_) * 8 + r_ % 8;                                    \
-        if (p >= (NPAN)) return;                                           \
+        pb_ = pb_ / (8 * GS_) * 8 + r_ % 8;                                \
+        if (pb_ >= (NPAN)) return;                                         \
     }                                                                      \
+    const int p = (PEXPR);                                                 \
     const int u = threadIdx.x;                                             \
     const int cl_ = u % (G), t = u / (G);                                  \
     const int c = hh_ * (G) + cl_;                                         \
@@ -261,7 +264,7 @@ __device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ 
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
-    VK_COL_SETUP_G(C, G, blk_, g.Pp)
+    VK_COL_SETUP_G(C, G, blk_, pb_, g.Pp)
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = Hrho[hoff<C>(p, t + T * s, c)];
