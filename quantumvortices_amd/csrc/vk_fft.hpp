@@ -18,6 +18,25 @@
 
 namespace vk {
 
+// Workgroup barrier that orders LDS only.  The transforms exchange data
+// through LDS alone, so their barriers need not wait for this thread's
+// outstanding global loads and stores (a plain __syncthreads() waits for
+// all of them, which serialises stores, and asynchronous global->LDS
+// copies, into every exchange).
+#ifndef VK_LDS_BARRIER
+#define VK_LDS_BARRIER 1
+#endif
+__device__ __forceinline__ void lds_barrier()
+{
+#if VK_LDS_BARRIER
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#else
+    __syncthreads();
+#endif
+}
+
 typedef double2 cplx;
 
 __device__ __forceinline__ cplx mk(double a, double b) { return make_double2(a, b); }
@@ -208,10 +227,10 @@ struct Stage {
 #pragma unroll
                     for (int r = 0; r < R; ++r) re[lpad(idxD + r * NS)] = v[a + (S / R) * r].x;
                 }
-                __syncthreads();
+                lds_barrier();
 #pragma unroll
                 for (int s = 0; s < S; ++s) v[s].x = re[lpad(t + T * s)];
-                __syncthreads();
+                lds_barrier();
 #pragma unroll
                 for (int a = 0; a < S / R; ++a) {
                     const int j = t + T * a;
@@ -219,10 +238,10 @@ struct Stage {
 #pragma unroll
                     for (int r = 0; r < R; ++r) re[lpad(idxD + r * NS)] = v[a + (S / R) * r].y;
                 }
-                __syncthreads();
+                lds_barrier();
 #pragma unroll
                 for (int s = 0; s < S; ++s) v[s].y = re[lpad(t + T * s)];
-                __syncthreads();
+                lds_barrier();
             } else {
 #pragma unroll
                 for (int a = 0; a < S / R; ++a) {
@@ -234,10 +253,10 @@ struct Stage {
                         im[lpad(idxD + r * NS)] = v[a + (S / R) * r].y;
                     }
                 }
-                __syncthreads();
+                lds_barrier();
 #pragma unroll
                 for (int s = 0; s < S; ++s) v[s] = mk(re[lpad(t + T * s)], im[lpad(t + T * s)]);
-                __syncthreads();
+                lds_barrier();
             }
             Stage<N, S, ST + 1, NS * R, SPLIT>::run(v, re, im, t, tw);
         }

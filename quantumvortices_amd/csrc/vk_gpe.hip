@@ -88,6 +88,8 @@ struct vk_gpe {
     cplx *K0 = nullptr, *K1 = nullptr, *K2 = nullptr, *Hpsi = nullptr, *Hrho = nullptr, *HN = nullptr;
     cplx* Xs = nullptr;   // x-space psi rows written by P1 and read by P3 (xpsi)
     int xpsi = 0;
+    int ncu = 0;        // compute units of the device
+    int p1pf = 0;       // RK4 P1 as the persistent prefetching kernel (4096^2)
     // advective ARGLE (Taylor-Green and/or counterflow); Hdy = IFFT_y(i ky kpsi)
     int adv_tg = 0;
     double adv_vx = 0.0, adv_vy = 0.0;
@@ -240,6 +242,36 @@ void p2_launch(vk_gpe* h, A... args)
 {
     constexpr int G = p2_cols<C>();
     colg_launch<C, G>(h, PK_P2, h->geo.Pp, vk::k_p2<C, G>, args...);
+}
+
+// The persistent prefetching P1 (vk::k_p1xp) runs at 4096^2 when the
+// runtime panel geometry is the one its LDS slot map was built for and the
+// row blocks split evenly over one workgroup per CU in XCD groups of 8.
+#ifndef VK_P1_PREFETCH
+#define VK_P1_PREFETCH 1
+#endif
+int p1_prefetch_ok(const vk_gpe* h)
+{
+    if (!VK_P1_PREFETCH || h->N != 4096 || !h->xpsi) return 0;
+    using C = CfgN<4096>;
+    static_assert(vk::p1p_lds<C>() <= 160 * 1024, "prefetch buffer exceeds LDS");
+    const int g8 = h->ncu / 8, r8 = (C::N / C::TR) / 8;
+    return h->ncu > 0 && h->ncu % 8 == 0 && r8 % g8 == 0 && h->geo.Pp == vk::geo_pp<C>() &&
+           h->geo.Pn == vk::geo_pn<C>();
+}
+
+template <class C>
+void p1p_launch(vk_gpe* h, double sc)
+{
+    if constexpr (C::N == 4096) {
+        auto k = vk::k_p1xp<C>;
+        const size_t lds = vk::p1p_lds<C>();
+        set_lds(k, lds);
+        ev_begin(h);
+        hipLaunchKernelGGL(k, dim3(h->ncu), dim3(C::ROWT), lds, h->stream, (const cplx*)h->Hpsi, h->Hrho,
+                           (const cplx*)h->tw, h->geo, sc, h->Xs);
+        ev_end(h, PK_P1);
+    }
 }
 
 int ncols(const vk_gpe* h, int pmode)
@@ -461,7 +493,9 @@ int rk4_step(vk_gpe* h)
         a.omega = h->p.omega;
         for (int stage = 1; stage <= 4; ++stage) {
             if (parts && !rs) particle_kv(h, Kp);
-            if (h->xpsi)
+            if (h->p1pf)
+                p1p_launch<C>(h, sc);
+            else if (h->xpsi)
                 row_launch<C>(h, PK_P1, vk::k_p1x<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc,
                               h->Xs);
             else
@@ -733,6 +767,13 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
             return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (fields)"));
     }
     h->xpsi = N >= VK_XPSI_MIN_N;
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "device attribute query failed"));
+        h->p1pf = p1_prefetch_ok(h);
+    }
     if (h->xpsi && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
     if (hipMalloc(&h->tw, sizeof(cplx) * N) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc tw"));
     // twiddles w_k = exp(-2 pi i k/N): long-double sin/cos on the first

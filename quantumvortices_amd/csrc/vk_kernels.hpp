@@ -204,22 +204,13 @@ __global__ void k_dealias(cplx* __restrict__ K, int N)
 // Fused RK4 / ARGLE passes
 // ----------------------------------------------------------------------------
 
-// P1 (row): psi row from Hpsi (inverse along x, times 1/N^2), rho = |psi|^2,
-// forward along x, keep kx >= 0 retained panels (rho is real).  XPSI: also
-// store the psi row in x-space (Xpsi[y*N + x]) for P3, which then needs no
-// transform of its own for psi.
-template <class C, bool XPSI = false>
-__device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
-                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
-                                                double scale, cplx* __restrict__ Xpsi = nullptr)
+// P1 after its loads: v holds the conjugated Hpsi row points of this thread
+template <class C, bool XPSI>
+__device__ __forceinline__ void p1_core(cplx (&v)[C::S], double* lg, int t, int y, int pidx0, int c,
+                                        cplx* __restrict__ Hrho, const cplx* __restrict__ tw, const Geo& g,
+                                        double scale, cplx* __restrict__ Xpsi)
 {
-    VK_ROW_SETUP(C)
-    cplx v[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int pp = VK_PANEL(s);
-        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
-    }
+    constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC;
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
     if constexpr (XPSI) {
         cplx* xr = Xpsi + (size_t)y * N;
@@ -239,6 +230,25 @@ __device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restr
     }
 }
 
+// P1 (row): psi row from Hpsi (inverse along x, times 1/N^2), rho = |psi|^2,
+// forward along x, keep kx >= 0 retained panels (rho is real).  XPSI: also
+// store the psi row in x-space (Xpsi[y*N + x]) for P3, which then needs no
+// transform of its own for psi.
+template <class C, bool XPSI = false>
+__device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+                                                const cplx* __restrict__ tw, Geo g, int psi_pmode,
+                                                double scale, cplx* __restrict__ Xpsi = nullptr)
+{
+    VK_ROW_SETUP(C)
+    cplx v[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int pp = VK_PANEL(s);
+        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
+    }
+    p1_core<C, XPSI>(v, lg, t, y, pidx0, c, Hrho, tw, g, scale, Xpsi);
+}
+
 template <class C>
 __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
                                                 const cplx* __restrict__ tw, Geo g, int psi_pmode,
@@ -253,6 +263,118 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1x(const cplx* __restrict__
                                                  double scale, cplx* __restrict__ Xpsi)
 {
     p1_body<C, true>(blockIdx.x, gridDim.x, Hpsi, Hrho, tw, g, psi_pmode, scale, Xpsi);
+}
+
+// Retained-panel geometry of a size, at compile time (as the host computes
+// it), and the LDS slots of the prefetching P1: the points s of a thread
+// whose panel can be stored get a 16-byte slot per lane and wave; points
+// whose stored lanes are disjoint share a slot (at 4096^2: 11 slots for the
+// 12 point indices with stored panels).
+struct P1Slots {
+    int slot[16];
+    int nslot;
+};
+template <class C> constexpr int geo_pp()
+{
+    constexpr int M = C::N / 3, pp = (M + 1 + C::TC - 1) / C::TC, pn = (M + C::TC - 1) / C::TC;
+    return pp + pn > C::NP ? C::NP : pp;
+}
+template <class C> constexpr int geo_pn()
+{
+    constexpr int M = C::N / 3, pp = (M + 1 + C::TC - 1) / C::TC, pn = (M + C::TC - 1) / C::TC;
+    return pp + pn > C::NP ? 0 : pn;
+}
+template <class C> constexpr P1Slots p1_slots()
+{
+    static_assert(C::S <= 16 && C::T / C::TC <= 128, "slot map sized for S <= 16, T/TC <= 128");
+    P1Slots m{};
+    unsigned long long occ[16][2] = {};
+    m.nslot = 0;
+    for (int s = 0; s < C::S; ++s) {
+        unsigned long long b[2] = {0ull, 0ull};
+        bool any = false;
+        for (int q = 0; q < C::T / C::TC; ++q) {
+            const int p = q + (C::T / C::TC) * s;
+            if (p < geo_pp<C>() || p >= C::NP - geo_pn<C>()) {
+                b[q / 64] |= 1ull << (q % 64);
+                any = true;
+            }
+        }
+        m.slot[s] = -1;
+        if (!any) continue;
+        for (int k = 0; k < m.nslot; ++k)
+            if (!(occ[k][0] & b[0]) && !(occ[k][1] & b[1])) {
+                m.slot[s] = k;
+                break;
+            }
+        if (m.slot[s] < 0) m.slot[s] = m.nslot++;
+        occ[m.slot[s]][0] |= b[0];
+        occ[m.slot[s]][1] |= b[1];
+    }
+    return m;
+}
+template <class C> constexpr size_t p1p_lds()
+{
+    return (size_t)C::TR * C::ROWRS * sizeof(double) + (size_t)p1_slots<C>().nslot * C::ROWT * sizeof(cplx);
+}
+
+// P1 of an RK4 stage (retained panels) on the psi-row path, persistent: each
+// workgroup walks row blocks of its XCD's contiguous range (XCD x = block %
+// 8 owns [x R/8, (x+1) R/8), its workgroups stride through it) and loads the
+// next block's Hpsi points into LDS with asynchronous global->LDS copies
+// while it transforms the current one.  Every workgroup runs the same fixed
+// number of iterations (the host checks that the blocks divide evenly).
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) k_p1xp(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+                                                  const cplx* __restrict__ tw, Geo g, double scale,
+                                                  cplx* __restrict__ Xpsi)
+{
+    constexpr P1Slots SM = p1_slots<C>();
+    constexpr int N = C::N, S = C::S, T = C::T, TC = C::TC, TR = C::TR, NP = C::NP;
+    constexpr int NW = C::ROWT / 64;
+    static_assert(C::ROWT % 64 == 0, "whole waves");
+    extern __shared__ double lds[];
+    const int u = threadIdx.x;
+    const int q_ = u % (TR * TC);
+    const int rr = q_ / TC, c = q_ % TC, pidx0 = u / (TR * TC);
+    const int t = TC * pidx0 + c;
+    double* lg = lds + rr * C::ROWRS;
+    cplx* buf = reinterpret_cast<cplx*>(lds + TR * C::ROWRS);
+    const int lane = u % 64, wv = u / 64;
+    const int R8 = (N / TR) / 8, G8 = gridDim.x / 8;
+    const int xcd = blockIdx.x % 8, j = blockIdx.x / 8;
+    const int K = R8 / G8;
+    auto prefetch = [&](int blk) {
+        const int yy = blk * TR + rr;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (SM.slot[s] < 0) continue;
+            const int pp = pidx0 + (T / TC) * s;
+            if (pp < g.Pp || pp >= NP - g.Pn)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(Hpsi + hoff<C>(pp, yy, c)),
+                    (__attribute__((address_space(3))) void*)(buf + (SM.slot[s] * NW + wv) * 64), 16, 0, 0);
+        }
+    };
+    prefetch(xcd * R8 + j);
+    for (int k = 0; k < K; ++k) {
+        const int y = (xcd * R8 + j + k * G8) * TR + rr;
+        // this wave's copies have landed (vmcnt), and every wave's (barrier)
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        cplx v[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int pp = pidx0 + (T / TC) * s;
+            v[s] = (SM.slot[s] >= 0 && (pp < g.Pp || pp >= NP - g.Pn))
+                       ? conjc(buf[(SM.slot[s] * NW + wv) * 64 + lane])
+                       : czero();
+        }
+        // all reads are done before the next copies overwrite the slots
+        __syncthreads();
+        if (k + 1 < K) prefetch(xcd * R8 + j + (k + 1) * G8);
+        p1_core<C, true>(v, lg, t, y, pidx0, c, Hrho, tw, g, scale, Xpsi);
+    }
 }
 
 // P2 (column, kx >= 0 retained panels): forward along y, dealias
