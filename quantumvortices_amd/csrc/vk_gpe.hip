@@ -12,6 +12,7 @@
 #include "vikshobha.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <algorithm>
 #include <array>
@@ -253,6 +254,9 @@ void p2_launch(vk_gpe* h, A... args)
 int p1_prefetch_ok(const vk_gpe* h)
 {
     if (!VK_P1_PREFETCH || h->N != 4096 || !h->xpsi) return 0;
+    // VK_NO_P1_PREFETCH=1 selects the plain P1 (tests compare the two)
+    const char* off = std::getenv("VK_NO_P1_PREFETCH");
+    if (off && off[0] == '1') return 0;
     using C = CfgN<4096>;
     static_assert(vk::p1p_lds<C>() <= 160 * 1024, "prefetch buffer exceeds LDS");
     const int g8 = h->ncu / 8, r8 = (C::N / C::TR) / 8;

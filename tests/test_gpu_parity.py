@@ -385,3 +385,29 @@ def test_particles_argle_vs_oracle(G, oracle_mod, ppotd):
     assert e < TOL
     assert abs(r - ro) <= 1e-9 * abs(ro)
     g.close()
+
+
+def test_prefetching_p1_equals_plain_p1(monkeypatch):
+    """At 4096^2 the RK4 P1 runs as a persistent kernel that prefetches the
+    next row pair into LDS; VK_NO_P1_PREFETCH=1 selects the plain row pass.
+    Both compute the same transforms, so 5 RK4 steps with particles agree to
+    roundoff (the two kernels may contract multiply-adds differently)."""
+    from quantumvortices_amd import gpe as G
+    N = 4096
+    say("case prefetching P1 vs plain P1 at 4096^2")
+    kw = dict(gp_N_obj=8, ppotV0=10.0, ppotd=2.0, gp_mass_part=5.0)
+    xs, ys, qs = G.random_vortices(64, seed=5)
+    pos = np.random.default_rng(6).uniform(0, 2 * np.pi, (8, 2))
+    out = []
+    for off in ("0", "1"):
+        monkeypatch.setenv("VK_NO_P1_PREFETCH", off)
+        g = G.GPE(G.params_from_namelist(N, 2.5e-4, 2.5, **kw))
+        g.set_ic_vortices(xs, ys, qs, nimg=3)
+        g.initialize_particles(pos)
+        g.gpe_rk4(5)
+        out.append((g.psi, g.particles()[0]))
+        g.close()
+    e = rel_l2(out[0][0], out[1][0])
+    ep = rel_l2(out[0][1], out[1][1])
+    say("  rel-L2 psi %.3e  positions %.3e" % (e, ep))
+    assert e < 1e-14 and ep < 1e-14
