@@ -529,13 +529,11 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
-    // XPSI: f stays in v[s].x and rho_d moves to v[s].y (psi needs no
-    // transform here, so v is free); otherwise f is parked in LDS slots after
-    // the transform regions while psi is transformed.  Particle positions
-    // are staged after those during the first transform, read back as
-    // broadcasts.
-    constexpr bool FV = XPSI;
-    double* qlds = lds + TR * C::ROWRS + (FV ? 0 : S * C::ROWT);
+    // f in registers (XPSI without real-space particles) or parked in LDS
+    // slots after the transform regions; particle positions staged after
+    // those during the first transform, read back as broadcasts
+    constexpr bool FREG = XPSI && (flags & 2) == 0;
+    double* qlds = lds + TR * C::ROWRS + (FREG ? 0 : S * C::ROWT);
     if constexpr ((flags & 2) != 0) {
         for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
     }
@@ -557,20 +555,23 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
     // v[s].x = rho_d (unnormalised); v[s].y = -V (unnormalised, flags & 1).
-    // The real factor f = g rho_d + V.
+    // The real factor f = g rho_d + V goes to thread-private LDS slots (after
+    // the transform regions) and waits there while psi is transformed.
     double* fl = lds + TR * C::ROWRS;
-#define VK_F(s) (FV ? v[s].x : fl[(s) * C::ROWT + threadIdx.x])
-#define VK_RHO(s) (FV ? v[s].y : v[s].x)
+    double fr_[FREG ? S : 1];
+#define VK_F(s) (FREG ? fr_[FREG ? (s) : 0] : fl[(s) * C::ROWT + threadIdx.x])
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         double fv;
         if constexpr ((flags & 1) != 0) fv = gs * v[s].x + vs * (-v[s].y);
         else fv = gs * v[s].x;
-        if constexpr (FV) v[s] = mk(fv, v[s].x);
-        else fl[s * C::ROWT + threadIdx.x] = fv;
+        VK_F(s) = fv;
     }
     if constexpr ((flags & 2) != 0) {
-        // real-space particle potential added into f; force partials from rho_d
+        // real-space particle potential added into the slots; force partials
+        double rho[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) rho[s] = v[s].x;
         const int blk = xcd_remap(blk_, nblk_);
         const int yb = blk * TR;
         const double inv2d2 = 1.0 / (2.0 * (pa.d * pa.d)), invd2 = 1.0 / (pa.d * pa.d);
@@ -625,14 +626,14 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
                         mask &= mask - 1;
                         double rs = 0.0;
 #pragma unroll
-                        for (int s = 0; s < S; ++s) rs = (s == sw) ? VK_RHO(s) : rs;
+                        for (int s = 0; s < S; ++s) rs = (s == sw) ? rho[s] : rs;
                         double X, dX;
                         pcoord(pwrap((t + T * sw) * pa.dx - qx), X, dX);
                         const double e = pa.V0 * exp(-(X * X + Y * Y) * inv2d2);
-                        if constexpr (FV) {
+                        if constexpr (FREG) {
 #pragma unroll
                             for (int s = 0; s < S; ++s)
-                                if (s == sw) v[s].x += vsign * e;
+                                if (s == sw) fr_[FREG ? s : 0] += vsign * e;
                         } else {
                             fl[sw * C::ROWT + threadIdx.x] += vsign * e;
                         }
@@ -685,7 +686,6 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
         }
     }
 #undef VK_F
-#undef VK_RHO
     if constexpr (ADV) {
         // -i (u + v).grad psi, one gradient component at a time
         cplx w[S];
