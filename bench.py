@@ -11,10 +11,11 @@ stream.
 
 Also reported: the pure-GPE (no particle) rate on the same field, the
 per-kernel roofline of the dominant pass (algorithmic bytes / measured
-average duration, against the 8 TB/s HBM peak), and the CPU baseline: the
-plain-C oracle (same algorithm as the reference, 16 unfused 2D FFTs per step
-plus the per-particle transforms) timed on a bounded sample of the same
-workload on the host cores; the batched ensemble; and ARGLE time-to-1e-10
+average duration, against the 8 TB/s HBM peak), the step-level roofline
+(compulsory bytes of all passes per step, and the counter-measured bytes),
+and the CPU baseline: the plain-C oracle (same algorithm as the reference,
+16 unfused 2D FFTs per step plus the per-particle transforms) timed on
+bounded samples of the same workload on the host cores, one thread and all; the batched ensemble; and ARGLE time-to-1e-10
 at 512^2 with the learned warm start against plain ARGLE from the Pade
 ansatz (and against the Thomas-Fermi start); and the spatio-temporal
 spectra at the reference's CompSTS size (128^2, 2048 snapshots).
@@ -37,38 +38,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def displaced_mass(N, V0, d, g):
-    """5x mass uses the displaced fluid of one particle, estimated from the
-    Thomas-Fermi density rho = max(0, 1 - V/g) of the particle potential
-    (src/gpe_particle.f90:49-54) on the grid."""
-    import numpy as np
-    L = 2 * math.pi
-    dx = L / N
-    n = int(min(N, 2 * math.ceil(8 * d / dx) + 1))
-    x = (np.arange(n) - n // 2) * dx
-    X, Y = np.meshgrid(x, x)
-    V = V0 * np.exp(-(X * X + Y * Y) / (2 * d * d))
-    rho = np.maximum(0.0, 1.0 - V / g)
-    return float(np.sum(1.0 - rho) * dx * dx)
-
-
-def make_workload(G, N, dt, particles, seed_v=3, seed_p=4, nv=256, npart=32):
-    import numpy as np
-    xs, ys, qs = G.random_vortices(nv, seed_v)
-    kw = {}
-    pos = None
-    if particles:
-        p0 = G.params_from_namelist(N, dt, 2.5)
-        mass = 5.0 * displaced_mass(N, 10.0 * p0.g, 2.0 * p0.xi, p0.g)
-        kw = dict(gp_N_obj=npart, ppotV0=10.0, ppotd=2.0, gp_mass_part=mass)
-        rng = np.random.default_rng(seed_p)
-        pos = rng.uniform(0.0, 2 * math.pi, (npart, 2))
-    p = G.params_from_namelist(N, dt, 2.5, **kw)
-    g = G.GPE(p)
-    g.set_ic_vortices(xs, ys, qs, nimg=3, sync=True)
-    if particles:
-        g.initialize_particles(pos)
-    return g, (xs, ys, qs, pos, kw)
+def make_workload(G, N, dt, particles, mass=None):
+    """The 4096^2 workload (quantumvortices_amd.workloads): 256 random-sign
+    vortices (seed 3) and, with particles, 32 particles (seed 4) of mass 5x
+    the fluid one particle displaces in its ARGLE ground state."""
+    from quantumvortices_amd import workloads as W
+    g, info = W.particle_workload(N, dt, npart=32 if particles else 0, mass=mass)
+    return g, info
 
 
 def time_steps(torch, g, steps, warmup):
@@ -85,44 +61,95 @@ def time_steps(torch, g, steps, warmup):
     return e0.elapsed_time(e1)
 
 
-def cpu_baseline(G, N, dt, wl, threads, psi_gpu):
-    """Oracle (same algorithm as the reference) on the host: one gpe_rhs
-    (3 FFTs + pointwise + the 32-particle potential translation), one psi
-    inverse FFT and one per-particle force (4 FFTs + short-range + x-sum)
-    at 4096^2, extrapolated to a full RK4 step:
-        t_step = 4 * (t_rhs + t_ifft + np * t_force)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(G, N, dt, info, threads, psi_gpu):
+    """The oracle (the reference's algorithm: 16 unfused 2D FFTs per RK4
+    step, per-particle spectral translation and forces) on the host cores,
+    on bounded samples of the workloads:
+      * 4096^2, `threads` threads: K = 2 full pure-GPE RK4 steps (measured),
+        one translation of the 32-particle potential and one particle force
+        (measured); the particle step is the pure step plus, per stage, the
+        translation and 32 forces (src/gpe.f90:498-671 calls the translation
+        once per right-hand side, src/gpe_particle.f90:429-531 one force per
+        particle per stage);
+      * 4096^2, one thread: one right-hand side and one inverse FFT,
+        extrapolated to a pure step as 4 x (rhs + ifft);
+      * 128^2 (the reference's own CPU-runnable case, mpiexec -n 1): 200 full
+        RK4 steps on one thread and on `threads` threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
+    from quantumvortices_amd import workloads as W
+    kw = info["kw"]
+    npart = kw.get("gp_N_obj", 0)
+    op = O.make_params(N, dt, W.XI_IN, np_=npart, V0_in=W.V0_IN, d_in=W.D_IN, mass=kw.get("gp_mass_part", 1.0))
+    legs = {}
     O.set_threads(threads)
-    xs, ys, qs, pos, kw = wl
-    op = O.make_params(N, dt, 2.5, np_=kw.get("gp_N_obj", 0), V0_in=10.0, d_in=2.0,
-                       mass=kw.get("gp_mass_part", 1.0))
+    # pure GPE: full steps
+    pure = O.State(O.make_params(N, dt, W.XI_IN))
+    pure.set_psi(psi_gpu, sync=True)
+    K = 2
+    t0 = time.perf_counter()
+    pure.rk4(K)
+    t_pure = (time.perf_counter() - t0) / K
+    legs["4096_pure_threads"] = {"sec_per_step": round(t_pure, 3), "steps_timed": K, "threads": threads}
+    # particle parts
     st = O.State(op)
-    st.psi[...] = psi_gpu
-    O.lib().orc_sync_psi(st.h)
-    npart = op.np
-    if npart:
-        st.part(0)[...] = pos
-        st.particle_potential()
+    st.set_psi(psi_gpu, sync=True)
+    st.part(0)[...] = info["pos"]
+    st.particle_potential()
     t0 = time.perf_counter()
-    st.rhs()
-    t_rhs = time.perf_counter() - t0
+    st.translate_all()
+    t_tr = time.perf_counter() - t0
+    st.shortrange()
     t0 = time.perf_counter()
-    O.fft2d(st.kpsi, +1)
-    t_ifft = time.perf_counter() - t0
-    t_force = 0.0
-    if npart:
+    st.particle_force(0)
+    t_force = time.perf_counter() - t0
+    t_part = t_pure + 4.0 * (t_tr + npart * t_force)
+    legs["4096_particles_threads"] = {"sec_per_step": round(t_part, 2), "threads": threads,
+                                      "translate_sec": round(t_tr, 3), "force_sec": round(t_force, 3)}
+    del st
+    # one thread, pure GPE
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    pure.rhs()
+    t_rhs1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.fft2d(pure.kpsi, +1)
+    t_ifft1 = time.perf_counter() - t0
+    legs["4096_pure_1thread"] = {"sec_per_step": round(4.0 * (t_rhs1 + t_ifft1), 2), "threads": 1,
+                                 "rhs_sec": round(t_rhs1, 3), "ifft_sec": round(t_ifft1, 3)}
+    del pure
+    # the reference's CPU-runnable case: 128^2, 4x4 vortex lattice, dt = 1e-3
+    n1 = 128
+    p1 = G.params_from_namelist(n1, 1e-3, W.XI_IN)
+    xs, ys, qs = G.lattice_vortices(4)
+    psi1 = G.ic_vortices_host(n1, xs, ys, qs, p1.xi)
+    for thr in (1, threads):
+        O.set_threads(thr)
+        s1 = O.State(O.make_params(n1, 1e-3, W.XI_IN))
+        s1.set_psi(psi1, sync=True)
         t0 = time.perf_counter()
-        st.particle_force(0)
-        t_force = time.perf_counter() - t0
-    t_step = 4.0 * (t_rhs + t_ifft + npart * t_force)
-    sample = ("oracle at %d^2 on %d threads: 1 gpe_rhs (%.2fs), 1 inverse FFT (%.2fs)%s; "
-              "per step = 4*(rhs+ifft+np*force) = %.1fs"
-              % (N, threads, t_rhs, t_ifft,
-                 (", 1 particle force (%.2fs) x %d particles" % (t_force, npart)) if npart else "",
-                 t_step))
-    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": "port", "sample": sample}
+        s1.rk4(200)
+        legs["128_%s" % ("1thread" if thr == 1 else "threads")] = {
+            "steps_per_s": round(200 / (time.perf_counter() - t0), 1), "steps_timed": 200, "threads": thr}
+    sample = ("oracle at %d^2: %d full pure-GPE RK4 steps on %d threads (%.2f s/step), plus one "
+              "32-particle potential translation (%.2f s) and one particle force (%.2f s) measured and "
+              "extrapolated to 4 x (translation + %d forces) per step -> %.1f s/step"
+              % (N, K, threads, t_pure, t_tr, t_force, npart, t_part))
+    return {"value": float("%.4g" % (1.0 / t_part)), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": sample, "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "pure_gpe_steps_per_s": float("%.4g" % (1.0 / t_pure)),
+            "legs": legs}
 
 
 def ensemble_bench(G, torch, steps, warmup):
@@ -349,9 +376,17 @@ def main():
             "bytes_per_launch": pb[dom], "avg_ms": pt[dom]}
     per_pass = {k: {"ms": round(pt[k], 4), "GB/s": round(pb[k] / (pt[k] * 1e-3) / 1e9, 1) if pt[k] > 0 else None}
                 for k in passes}
-    # step-level view against the naive compulsory-traffic roofline
-    # (53 full-field transfers per step, 848 N^2 bytes)
-    step_bytes_ref = 848.0 * N * N
+    # step-level roofline: the compulsory bytes of the four passes of the
+    # four stages (the particle kernels move a few MB), and the bytes the
+    # counters measured for the same passes
+    step_bytes = 4.0 * sum(pb[k] for k in passes)
+    meas = [measured_traffic(k, N) for k in passes]
+    step_meas = 4.0 * sum(meas) if all(m is not None for m in meas) else None
+    step_gbs = step_bytes * steps_per_s / 1e9
+    step_roof = {"bound": "hbm", "compulsory_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                 "measured_bytes_per_step": step_meas,
+                 "measured_frac": round(step_meas * steps_per_s / 1e9 / HBM_PEAK_GBS, 4) if step_meas else None}
 
     ens = None
     if not args.no_ensemble:
@@ -373,7 +408,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the host cores this process may use, at most the 16 a one-GPU box
+        # grants a job (OMP_NUM_THREADS there)
+        share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+        threads = args.cpu_threads or max(1, min(len(os.sched_getaffinity(0)), share))
         log("bench: CPU baseline (oracle) with %d threads" % threads)
         cpu = cpu_baseline(G, N, dt, wl, threads, psi_for_cpu)
         log("bench: cpu", cpu)
@@ -396,11 +434,13 @@ def main():
         "data": "synthetic (seeded vortex gas + particles; no dataset)",
         "config": {"workload": "GPE RK4 4096^2 FP64, dt=2.5e-4, 256 random-sign vortices (seed 3), "
                                "32 inertial particles (seed 4), V0=10g, d=2xi, 2/3 dealias",
-                   "N": N, "dt": dt, "vortices": 256, "particles": 32, "parallelism": "single GPU"},
+                   "N": N, "dt": dt, "vortices": 256, "particles": 32, "particle_mass": wl["mass"],
+                   "particle_mass_rule": "5 x integral of (1 - rho) of one particle's ARGLE ground state",
+                   "parallelism": "single GPU"},
         "roofline": roof,
         "per_pass": per_pass,
         "pure_gpe_steps_per_s": round(pure, 3) if pure else None,
-        "effective_GBps_vs_53_transfer_step": round(step_bytes_ref * steps_per_s / 1e9, 1),
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
         "ensemble": ens,
         "argle_warm_start": argle,

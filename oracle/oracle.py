@@ -44,6 +44,10 @@ class OrcParams(ctypes.Structure):
         ("adv", ctypes.c_int),
         ("vnx", ctypes.c_double),
         ("vny", ctypes.c_double),
+        ("sgle", ctypes.c_int),
+        ("oneybeta", ctypes.c_double),
+        ("nuN", ctypes.c_double),
+        ("ktrunc", ctypes.c_double),
     ]
 
 
@@ -81,6 +85,10 @@ def lib():
         L.orc_rhs_only.argtypes = [ctypes.c_void_p, dp]
         L.orc_diagnostics.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.orc_particle_force.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.orc_set_noise.argtypes = [ctypes.c_void_p, dp]
+        L.orc_set_velocity.argtypes = [ctypes.c_void_p, dp, dp]
+        L.orc_omega.argtypes = [ctypes.c_void_p]
+        L.orc_omega.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -94,7 +102,8 @@ def set_threads(n):
 
 
 def make_params(N, dt, xi_in=2.5, rho_avg=1.0, disp=0.0, csound=None,
-                np_=0, V0_in=0.0, d_in=0.0, mass=1.0, force_dealias=1, adv=0, vnx=0.0, vny=0.0):
+                np_=0, V0_in=0.0, d_in=0.0, mass=1.0, force_dealias=1, adv=0, vnx=0.0, vny=0.0,
+                sgle=0, oneybeta=0.0, nuN=0.0, chempot0=None, ktrunc=0.0):
     """Grid + constants as the reference derives them (src/mod_grid.f90:63-101,
     src/driver.f90:360-393).  Default: g=1 with xi = xi_in*dx, i.e.
     csound = sqrt(2)*xi so that alpha = xi^2."""
@@ -112,6 +121,12 @@ def make_params(N, dt, xi_in=2.5, rho_avg=1.0, disp=0.0, csound=None,
     p.adv = adv
     p.vnx = vnx
     p.vny = vny
+    p.sgle = sgle
+    p.oneybeta = oneybeta
+    p.nuN = nuN
+    p.ktrunc = ktrunc
+    if sgle and chempot0 is not None:
+        p.omega = chempot0      # omega_chem = chempot0 under GPE_SGLE (src/driver.f90:373)
     return p
 
 
@@ -227,3 +242,17 @@ class State:
 
     def shortrange(self):
         lib().orc_shortrange(self.h)
+
+    def set_noise(self, noise):
+        a = np.ascontiguousarray(noise, dtype=np.complex128)
+        assert a.shape == (self.N, self.N)
+        lib().orc_set_noise(self.h, _dp(a))
+
+    def set_velocity(self, ux, uy):
+        ux = np.ascontiguousarray(ux, dtype=np.float64)
+        uy = np.ascontiguousarray(uy, dtype=np.float64)
+        lib().orc_set_velocity(self.h, _dp(ux), _dp(uy))
+
+    @property
+    def omega(self):
+        return lib().orc_omega(self.h)

@@ -165,15 +165,19 @@ void orc_fft2d(int N, const orc_c* in, orc_c* out, int dir)
     orc_c* tw = (orc_c*)malloc(sizeof(orc_c) * N);
     orc_c* tmp = (orc_c*)malloc(sizeof(orc_c) * n2);
     orc_twiddles(N, tw);
-    /* pass 1: transform along the contiguous axis of the input */
+    /* pass 1: transform along the contiguous axis of the input, B rows at a
+     * time, transposed on store (tmp[c][r]) in runs of B contiguous values */
+    const int B = N < 16 ? N : 16;
 #pragma omp parallel for num_threads(g_threads) schedule(static)
-    for (int r = 0; r < N; ++r) {
-        orc_c* row = (orc_c*)malloc(sizeof(orc_c) * N);
-        memcpy(row, in + (size_t)r * N, sizeof(orc_c) * N);
-        fft1d(row, N, tw, dir);
-        /* transpose on store: tmp[c][r] */
-        for (int c = 0; c < N; ++c) tmp[(size_t)c * N + r] = row[c];
-        free(row);
+    for (int r0 = 0; r0 < N; r0 += B) {
+        orc_c* rows = (orc_c*)malloc(sizeof(orc_c) * (size_t)B * N);
+        for (int b = 0; b < B; ++b) {
+            memcpy(rows + (size_t)b * N, in + (size_t)(r0 + b) * N, sizeof(orc_c) * N);
+            fft1d(rows + (size_t)b * N, N, tw, dir);
+        }
+        for (int c = 0; c < N; ++c)
+            for (int b = 0; b < B; ++b) tmp[(size_t)c * N + r0 + b] = rows[(size_t)b * N + c];
+        free(rows);
     }
     /* pass 2: transform along the other axis (now contiguous), stored in the
      * transposed orientation of the input, which is exactly the output
@@ -184,11 +188,11 @@ void orc_fft2d(int N, const orc_c* in, orc_c* out, int dir)
         orc_c* row = tmp + (size_t)c * N;
         fft1d(row, N, tw, dir);
     }
-    /* tmp[c][r] now holds: dir=-1: c=kx, r->ky ; dir=+1: c=y?  For dir=-1,
-     * input [y][x]: pass 1 along x gives row y, col kx -> tmp[kx][y];
-     * pass 2 along y -> tmp[kx][ky] = out layout.  For dir=+1, input
-     * [kx][ky]: pass 1 along ky -> tmp[y][kx]; pass 2 along kx ->
-     * tmp[y][x] = out layout. */
+    /* tmp[c][r] now holds: dir=-1, input [y][x]: pass 1 along x gives row y,
+     * col kx -> tmp[kx][y]; pass 2 along y -> tmp[kx][ky] = out layout.
+     * dir=+1, input [kx][ky]: pass 1 along ky -> tmp[y][kx]; pass 2 along
+     * kx -> tmp[y][x] = out layout. */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (size_t i = 0; i < n2; ++i)
         out[i] = (dir > 0) ? rmul(scale, tmp[i]) : tmp[i];
     free(tmp);
@@ -208,6 +212,8 @@ struct orc_state {
     double *pos, *vel, *f0ext, *frc, *tpos, *tvel, *kq[4], *ku[4], *sr;
     int sr_init;                 /* shortrange_repulsion has run */
     double gpdeltaE, gpr0obj;
+    orc_c* noise;                /* SGLE noise of the next step (x-space) */
+    double *ux, *uy;             /* advection velocity for adv = 2 */
 };
 
 static orc_c* calloc_c(size_t n) { return (orc_c*)calloc(n, sizeof(orc_c)); }
@@ -223,6 +229,8 @@ orc_state* orc_create(const orc_params* p)
     s->str1 = calloc_c(n2); s->str2 = calloc_c(n2); s->tmp1 = calloc_c(n2);
     s->tmp2 = calloc_c(n2); s->tmp3 = calloc_c(n2); s->cmpxtmp = calloc_c(n2);
     s->vobj_k = calloc_c(n2); s->trans_v = calloc_c(n2);
+    s->noise = calloc_c(n2);
+    s->ux = (double*)calloc(n2, sizeof(double)); s->uy = (double*)calloc(n2, sizeof(double));
     int np = p->np > 0 ? p->np : 1;
     s->pos = calloc(2 * np, sizeof(double)); s->vel = calloc(2 * np, sizeof(double));
     s->f0ext = calloc(2 * np, sizeof(double)); s->frc = calloc(2 * np, sizeof(double));
@@ -242,12 +250,23 @@ void orc_destroy(orc_state* s)
     free(s->psi); free(s->kpsi); free(s->nlin); free(s->str1); free(s->str2);
     free(s->tmp1); free(s->tmp2); free(s->tmp3); free(s->cmpxtmp);
     free(s->vobj_k); free(s->trans_v);
+    free(s->noise); free(s->ux); free(s->uy);
     free(s->pos); free(s->vel); free(s->f0ext); free(s->frc);
     free(s->tpos); free(s->tvel); free(s->sr);
     for (int i = 0; i < 4; ++i) { free(s->kq[i]); free(s->ku[i]); }
     free(s);
 }
 
+void orc_set_noise(orc_state* s, const orc_c* noise)
+{
+    memcpy(s->noise, noise, sizeof(orc_c) * (size_t)s->p.N * s->p.N);
+}
+void orc_set_velocity(orc_state* s, const double* ux, const double* uy)
+{
+    memcpy(s->ux, ux, sizeof(double) * (size_t)s->p.N * s->p.N);
+    memcpy(s->uy, uy, sizeof(double) * (size_t)s->p.N * s->p.N);
+}
+double orc_omega(const orc_state* s) { return s->p.omega; }
 orc_c* orc_psi(orc_state* s) { return s->psi; }
 orc_c* orc_kpsi(orc_state* s) { return s->kpsi; }
 orc_c* orc_vobj_k(orc_state* s) { return s->vobj_k; }
@@ -266,6 +285,14 @@ double* orc_part(orc_state* s, int which)
 #define FOR_K(s)                                                            \
     for (int i1 = 0; i1 < (s)->p.N; ++i1)                                   \
         for (int i2 = 0; i2 < (s)->p.N; ++i2)
+/* the same loop over independent modes (no reduction), split over threads */
+#define FOR_K_PAR(s)                                                        \
+    _Pragma("omp parallel for num_threads(g_threads) schedule(static)")      \
+    for (int i1 = 0; i1 < (s)->p.N; ++i1)                                   \
+        for (int i2 = 0; i2 < (s)->p.N; ++i2)
+#define FOR_X_PAR(n2)                                                       \
+    _Pragma("omp parallel for num_threads(g_threads) schedule(static)")      \
+    for (size_t i = 0; i < (n2); ++i)
 #define KIDX(s) ((size_t)i1 * (s)->p.N + i2)
 #define K1 (s->kwn[i1])
 #define K2 (s->kwn[i2])
@@ -280,7 +307,7 @@ void orc_sync_psi(orc_state* s)
  * dealias kpsi, then psi = IFFT(kpsi). */
 void orc_ic_sync(orc_state* s)
 {
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         if (orc_mask(&s->p, K1, K2)) s->kpsi[KIDX(s)] = 0;
     }
     orc_fft2d(s->p.N, s->kpsi, s->psi, +1);
@@ -298,6 +325,7 @@ void orc_particle_potential(orc_state* s)
     const double pi = 4.0 * atan(1.0);
     double c1 = 2 * pi / (2.0 * p->L);
     double c2 = 2 * pi / (2.0 * p->L);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int i2 = 0; i2 < N; ++i2) {
         double y = i2 * p->dx;
         for (int i1 = 0; i1 < N; ++i1) {
@@ -312,7 +340,7 @@ void orc_particle_potential(orc_state* s)
         }
     }
     orc_fft2d(N, s->cmpxtmp, s->cmpxtmp, -1);
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         if (orc_mask(p, K1, K2)) s->cmpxtmp[k] = 0;
         s->vobj_k[k] = s->cmpxtmp[k];
@@ -325,7 +353,7 @@ void orc_particle_potential(orc_state* s)
 void orc_translate_all(orc_state* s)
 {
     const orc_params* p = &s->p;
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         double kx = K1, ky = K2;
         double sr = 0.0, si = 0.0;
         for (int ip = 0; ip < p->np; ++ip) {
@@ -380,15 +408,15 @@ void orc_particle_force(orc_state* s, int ip, double* f)
     const orc_params* p = &s->p;
     const int N = p->N;
     size_t n2 = (size_t)N * N;
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         double th = (double)K1 * s->pos[2 * ip] + (double)K2 * s->pos[2 * ip + 1];
         s->trans_v[KIDX(s)] = cmul(mkc(cos(th), -sin(th)), s->vobj_k[KIDX(s)]);
     }
     orc_fft2d(N, s->trans_v, s->trans_v, +1);
     orc_shortrange(s);
-    for (size_t i = 0; i < n2; ++i) s->cmpxtmp[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
+    FOR_X_PAR(n2) s->cmpxtmp[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
     orc_fft2d(N, s->cmpxtmp, s->cmpxtmp, -1);
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         if (p->force_dealias && orc_mask(p, K1, K2)) s->cmpxtmp[k] = 0;
         orc_c c = s->cmpxtmp[k];
@@ -454,9 +482,9 @@ static void gpe_rhs(orc_state* s)
     orc_c num = mkc(1.0, -p->disp);
     orc_c pref = mkc(IM(num), -RE(num));  /* (a+bi)/i = b - ai */
     if (p->np > 0) orc_translate_all(s);
-    for (size_t i = 0; i < n2; ++i) s->nlin[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
+    FOR_X_PAR(n2) s->nlin[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
     orc_fft2d(N, s->nlin, s->nlin, -1);
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         double kx = K1, ky = K2;
         double kr2 = kx * kx + ky * ky;
@@ -464,13 +492,13 @@ static void gpe_rhs(orc_state* s)
         if (orc_mask(p, K1, K2)) s->nlin[k] = 0;
     }
     orc_fft2d(N, s->nlin, s->nlin, +1);
-    for (size_t i = 0; i < n2; ++i) {
+    FOR_X_PAR(n2) {
         orc_c v = cmul(rmul(p->g, s->nlin[i]), s->psi[i]);
         if (p->np > 0) v = cadd(v, cmul(s->trans_v[i], s->psi[i]));
         s->nlin[i] = v;
     }
     orc_fft2d(N, s->nlin, s->nlin, -1);
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         s->nlin[k] = cmul(pref, cadd(s->tmp1[k], s->nlin[k]));
         if (orc_mask(p, K1, K2)) s->nlin[k] = 0;
@@ -490,7 +518,7 @@ void orc_rk4_step(orc_state* s)
     const int N = p->N;
     const double dt = p->dt;
     const double oney6 = 1.0 / 6.0, half = 0.5, two = 2.0;
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         if (orc_mask(p, K1, K2)) s->kpsi[k] = 0;
         s->str1[k] = s->kpsi[k];
@@ -500,7 +528,7 @@ void orc_rk4_step(orc_state* s)
         gpe_rhs(s);
         if (p->np > 0) particle_rk4_stage(s, stage);
         double wacc = stage == 1 || stage == 4 ? dt * oney6 : dt * oney6 * two;
-        FOR_K(s) {
+        FOR_K_PAR(s) {
             size_t k = KIDX(s);
             orc_c r = s->nlin[k];
             s->str2[k] = cadd(s->str2[k], rmul(wacc, r));
@@ -516,8 +544,10 @@ void orc_rk4_step(orc_state* s)
     }
 }
 
-/* src/gpe.f90:5-224 (ARGLE without SGLE; with the advective terms of
- * GPE_ADVECTION and GPE_COUNTERFLOW when p->adv / p->vnx, p->vny are set).
+/* src/gpe.f90:5-224 (ARGLE; with the advective terms of GPE_ADVECTION and
+ * GPE_COUNTERFLOW when p->adv / p->vnx, p->vny are set, adv = 2 taking the
+ * velocity from orc_set_velocity in place of the Taylor-Green field; with
+ * the GPE_SGLE noise and chemical-potential update when p->sgle is set).
  * residual (not defined by the reference, which has no convergence test):
  *   ||kpsi_new - kpsi_old||_2 / (dt * ||kpsi_old||_2) over all modes. */
 void orc_argle_step(orc_state* s, double* residual)
@@ -530,7 +560,7 @@ void orc_argle_step(orc_state* s, double* residual)
     orc_fft2d(N, s->kpsi, s->psi, +1);
     if (p->adv || cflow) {
         /* gradients of psi: IFFT(i kx kpsi), IFFT(i ky kpsi) (:78-81,91-94) */
-        FOR_K(s) {
+        FOR_K_PAR(s) {
             size_t k = KIDX(s);
             s->tmp1[k] = cmul(mkc(0.0, (double)K1), s->kpsi[k]);
             s->tmp2[k] = cmul(mkc(0.0, (double)K2), s->kpsi[k]);
@@ -539,23 +569,24 @@ void orc_argle_step(orc_state* s, double* residual)
         orc_fft2d(N, s->tmp2, s->tmp2, +1);
     }
     if (p->np > 0) orc_translate_all(s);
-    for (size_t i = 0; i < n2; ++i) s->nlin[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
+    FOR_X_PAR(n2) s->nlin[i] = mkc(sq(cabs_f(s->psi[i])), 0.0);
     orc_fft2d(N, s->nlin, s->nlin, -1);
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         if (orc_mask(p, K1, K2)) s->nlin[KIDX(s)] = 0;
     }
     orc_fft2d(N, s->nlin, s->nlin, +1);
-    for (size_t i = 0; i < n2; ++i) {
+    FOR_X_PAR(n2) {
         orc_c v = cmul(rmul(-p->g, s->nlin[i]), s->psi[i]);
         if (p->np > 0) {
             orc_c pot = cmul(s->trans_v[i], s->psi[i]);
             v = cadd(v, mkc(-RE(pot), -IM(pot)));
         }
         if (p->adv) {
-            /* Taylor-Green velocity (:114-130):
+            /* Taylor-Green velocity (:114-130), or the supplied field:
              * -i (u.grad psi) - (1/4)(1/alpha)|u|^2 psi */
             const double x = (double)(i % N) * p->dx, y = (double)(i / N) * p->dx;
-            const double ux = sin(x) * cos(y), uy = -cos(x) * sin(y);
+            const double ux = p->adv == 2 ? s->ux[i] : sin(x) * cos(y);
+            const double uy = p->adv == 2 ? s->uy[i] : -cos(x) * sin(y);
             const orc_c a = cadd(rmul(ux, s->tmp1[i]), rmul(uy, s->tmp2[i]));
             v = csub(v, cmul(mkc(0.0, 1.0), a));
             v = csub(v, rmul(0.25 * (1.0 / p->alpha) * (ux * ux + uy * uy), s->psi[i]));
@@ -568,8 +599,26 @@ void orc_argle_step(orc_state* s, double* residual)
         s->nlin[i] = v;
     }
     orc_fft2d(N, s->nlin, s->nlin, -1);
-    double dnum = 0.0, dden = 0.0;
-    FOR_K(s) {
+    if (p->sgle) {
+        /* noise (:150-164): CMPLX(v1,v2)*(1/SQRT(dvol))*noisesigma, from the
+         * supplied standard normals */
+        const double noisesigma = sqrt((1.0 / (2.0 * p->alpha)) * p->oneybeta);
+        FOR_X_PAR(n2) s->cmpxtmp[i] = rmul(noisesigma, rmul(1.0 / sqrt(p->dvol), s->noise[i]));
+        /* finiteT_adhoc_chempot (:226-266) on psi = IFFT(kpsi) of this step:
+         * norm = sum |psi|^2 dvol; chempot -= dt (nuN/(Lx Ly)) (norm - Nparticle),
+         * Nparticle = Lx Ly (src/driver.f90:362) */
+        double norm = 0.0;
+        for (size_t i = 0; i < n2; ++i) norm = norm + sq(cabs_f(s->psi[i]));
+        norm = norm * p->dvol;
+        s->p.omega = p->omega - dt * (p->nuN / (p->L * p->L)) * (norm - p->L * p->L);
+        orc_fft2d(N, s->cmpxtmp, s->cmpxtmp, -1);
+    }
+    /* the update per mode (into tmp3), then the residual sums in the loop
+     * order of the update, then kpsi <- tmp3.  SGLE (:200-217): the noise
+     * term FFT(noise) sqrt(dt), dealiased, is added to the numerator; with
+     * ktrunc > 0 modes with ksqr > ktrunc^2 are zeroed (GPE_SELFTRUNC). */
+    const double sdt = sqrt(dt);
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         double kx = K1, ky = K2;
         double kr2 = kx * kx + ky * ky;
@@ -577,11 +626,22 @@ void orc_argle_step(orc_state* s, double* residual)
         orc_c t1 = cadd(s->nlin[k], rmul(p->omega, old));
         double t2 = 1.0 + p->alpha * kr2 * dt;
         orc_c num = cadd(old, rmul(dt, t1));
+        if (p->sgle) {
+            orc_c nt = rmul(sdt, s->cmpxtmp[k]);
+            if (orc_mask(p, K1, K2)) nt = 0;
+            num = cadd(num, nt);
+        }
         orc_c nw = mkc(RE(num) / t2, IM(num) / t2);
-        s->kpsi[k] = nw;
+        if (p->sgle && p->ktrunc > 0.0 && (double)(K1 * K1 + K2 * K2) > p->ktrunc * p->ktrunc) nw = 0;
+        s->tmp3[k] = nw;
+    }
+    double dnum = 0.0, dden = 0.0;
+    for (size_t k = 0; k < n2; ++k) {
+        orc_c old = s->kpsi[k], nw = s->tmp3[k];
         dnum += sq(RE(nw) - RE(old)) + sq(IM(nw) - IM(old));
         dden += sq(RE(old)) + sq(IM(old));
     }
+    memcpy(s->kpsi, s->tmp3, sizeof(orc_c) * n2);
     if (residual) *residual = sqrt(dnum) / (dt * sqrt(dden));
 }
 
@@ -602,7 +662,7 @@ void orc_diagnostics(orc_state* s, double* scal, double* spec, double* pscal)
     const double two = 2.0, half = 0.5, alpha = p->alpha;
     const double smlnum = (double)1e-30f;   /* :43, default-real literal */
     const double L2 = p->L * p->L;
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         s->tmp1[k] = cmul(mkc(0.0, (double)K1), s->kpsi[k]);
         s->tmp2[k] = cmul(mkc(0.0, (double)K2), s->kpsi[k]);
@@ -620,7 +680,7 @@ void orc_diagnostics(orc_state* s, double* scal, double* spec, double* pscal)
     momx = two * alpha * momx * p->dvol / L2;
     momy = two * alpha * momy * p->dvol / L2;
     orc_fft2d(N, s->nlin, s->nlin, -1);
-    FOR_K(s) {
+    FOR_K_PAR(s) {
         size_t k = KIDX(s);
         if (orc_mask(p, K1, K2)) s->nlin[k] = 0;
         s->str1[k] = s->nlin[k];
@@ -667,7 +727,7 @@ void orc_diagnostics(orc_state* s, double* scal, double* spec, double* pscal)
     }
     /* :265-317  w = 2 alpha Im(psi* grad psi)/(|psi|+smlnum),
      *           dabs = Re(psi* grad psi)/(|psi|+smlnum) */
-    for (size_t i = 0; i < n2; ++i) {
+    FOR_X_PAR(n2) {
         double rholoc = cabs_f(s->psi[i]) + smlnum;
         orc_c cp = conj(s->psi[i]);
         orc_c a1 = cmul(cp, s->tmp1[i]);

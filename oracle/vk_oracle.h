@@ -42,6 +42,14 @@ typedef struct {
      * term (GPE_COUNTERFLOW, namelist gpcounterflow) */
     int adv;
     double vnx, vny;
+    /* SGLE (GPE_SGLE, src/gpe.f90:149-217,226-266; namelist gpfiniteT,
+     * src/driver.f90:59): noise of strength noisesigma =
+     * sqrt((1/(2 alpha)) oneybeta) (src/driver.f90:376) and the chemical
+     * potential relaxed towards norm = L^2 at rate nuN; ktrunc > 0 zeroes
+     * kpsi beyond |k| = ktrunc (GPE_SELFTRUNC).  omega starts at chempot0
+     * and is updated in place. */
+    int sgle;
+    double oneybeta, nuN, ktrunc;
 } orc_params;
 
 typedef struct orc_state orc_state;
@@ -78,6 +86,13 @@ void orc_particle_potential(orc_state* s);
 void orc_translate_all(orc_state* s);
 void orc_particle_force(orc_state* s, int ip, double* f);
 void orc_shortrange(orc_state* s);
+/* x-space noise field for the next SGLE step: complex standard normals
+ * (v1 + i v2 per point, before the noisesigma/sqrt(dvol) scale) */
+void orc_set_noise(orc_state* s, const orc_c* noise);
+/* caller-supplied advection velocity (adv = 2): u = (ux, uy) per point,
+ * x-space, in place of the Taylor-Green field */
+void orc_set_velocity(orc_state* s, const double* ux, const double* uy);
+double orc_omega(const orc_state* s);
 
 #ifdef __cplusplus
 }

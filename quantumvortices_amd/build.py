@@ -22,21 +22,23 @@ FLAGS = [
 
 
 FORTRAN = os.environ.get("FC", "/opt/rocm/bin/amdflang")
-FSRC = [os.path.join(HERE, "fortran", "vikshobha_mod.f90"), os.path.join(HERE, "fortran", "driver_check.f90")]
+FMOD = os.path.join(HERE, "fortran", "vikshobha_mod.f90")
+FDRIVERS = {name: os.path.join(HERE, "fortran", name + ".f90") for name in ("driver_check", "driver_part")}
 FBIN = os.path.join(HERE, "lib", "driver_check")
 
 
 def build_fortran(verbose=False):
-    """ISO_C_BINDING layer + the reference-style check driver (amdflang)."""
+    """ISO_C_BINDING layer + the reference-style check drivers (amdflang)."""
     import tempfile
     with tempfile.TemporaryDirectory() as mod:
-        cmd = [FORTRAN, "-O2", "-J", mod, "-o", FBIN] + FSRC + [
-            "-L" + os.path.dirname(LIB), "-lvikshobha", "-Wl,-rpath,$ORIGIN"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("amdflang failed:\n" + r.stderr)
+        for name, src in FDRIVERS.items():
+            cmd = [FORTRAN, "-O2", "-J", mod, "-o", os.path.join(HERE, "lib", name), FMOD, src,
+                   "-L" + os.path.dirname(LIB), "-lvikshobha", "-Wl,-rpath,$ORIGIN"]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError("amdflang failed:\n" + r.stderr)
     return FBIN
 
 

@@ -389,6 +389,24 @@ vk::PartArgs part_args(const vk_gpe* h, int nrows, int half, int stage, int src 
     return a;
 }
 
+// Window radius (in the minimum-image displacement a) of the real-space
+// particle potential.  The potential depends on the periodic coordinate
+// X(a) = 2(s + s^3/6 + s^5/40), s = sin(a/2), which saturates at
+// X(pi) = 2.383 near the half box, so V at the edge a = 10 d is
+// V0 exp(-X(10 d)^2/(2 d^2)), not V0 exp(-50).  The 10 d window is kept when
+// that is below exp(-45) (3e-20) of the peak (X(a) = a - O(a^5) for small
+// a, so narrow potentials keep it); otherwise the window covers the whole
+// period (any radius >= pi does).
+double real_space_window(double d)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    const double a = std::min(10.0 * d, pi);
+    const double s = std::sin(0.5 * a), s2 = s * s;
+    const double X = 2.0 * (s + (1.0 / 6.0) * (s2 * s) + (1.0 / 40.0) * (s2 * s2 * s));
+    if (10.0 * d < pi && X * X / (2.0 * d * d) >= 45.0) return 10.0 * d;
+    return 4.0;
+}
+
 vk::P3Part p3part(const vk_gpe* h)
 {
     vk::P3Part q{};
@@ -400,7 +418,7 @@ vk::P3Part p3part(const vk_gpe* h)
         q.np = h->p.np;
         q.V0 = h->p.V0;
         q.d = h->p.d;
-        q.W = 10.0 * h->p.d;   // exp(-50) of the peak at the window edge
+        q.W = real_space_window(h->p.d);
         q.dx = 2.0 * pi / h->N;
     }
     return q;
@@ -1025,9 +1043,13 @@ int vk_gpe_particles_set(vk_gpe* h, const double* pos, const double* vel, const 
     if (np <= 0) return fail(VK_ERR_STATE, "particles_set: handle created with np = 0");
     int rc = need_x0(h);
     if (rc) return rc;
-    if (pos) HIPCHK(hipMemcpy(h->pst, pos, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
-    if (vel) HIPCHK(hipMemcpy(h->pst + 2 * np, vel, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
-    if (fext) HIPCHK(hipMemcpy(h->pst + 4 * np, fext, sizeof(double) * 2 * np, hipMemcpyHostToDevice));
+    // steps queued by vk_gpe_rk4_steps_async read and write the particle
+    // state (the side-stream updates are joined into h->stream at the end of
+    // every step): the copies go on h->stream, after them
+    if (pos) HIPCHK(hipMemcpyAsync(h->pst, pos, sizeof(double) * 2 * np, hipMemcpyDefault, h->stream));
+    if (vel) HIPCHK(hipMemcpyAsync(h->pst + 2 * np, vel, sizeof(double) * 2 * np, hipMemcpyDefault, h->stream));
+    if (fext) HIPCHK(hipMemcpyAsync(h->pst + 4 * np, fext, sizeof(double) * 2 * np, hipMemcpyDefault, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     if (!h->have_part && (rc = particle_potential(h))) return rc;
     h->kv_valid = 0;
     HIPCHK(hipStreamSynchronize(h->stream));

@@ -1,7 +1,8 @@
 ! A reference-style driver on the ISO_C_BINDING layer: sets up a 4x4 Padé
 ! vortex lattice at N=64, syncs it like the reference's restart path, runs
 ! RK4 steps and diagnostics through the reference's subroutine names, and
-! writes psi, kpsi and the energies to a stream file for comparison; then
+! writes psi, kpsi and the energies to a stream file (argument 1) for
+! comparison, the diagnostics' text files into argument 2; then
 ! takes Ntt = 32 spatio-temporal snapshots, one per RK4 step, and appends
 ! spec_sts2d and arr1.
 PROGRAM driver_check
@@ -10,16 +11,20 @@ PROGRAM driver_check
   USE vikshobha_gpu
   IMPLICIT NONE
   INTEGER, PARAMETER :: N = 64, nsteps = 10
-  REAL(C_DOUBLE) :: pi, dx, xi, csound, xs(16), ys(16), shift
+  REAL(C_DOUBLE) :: pi, dx, xi_l, cs_l, xs(16), ys(16)
   INTEGER(C_INT) :: qs(16), rc
   INTEGER :: i, j, k, u
-  CHARACTER(256) :: outname
+  CHARACTER(256) :: outname, outdir
 
   pi = 4.0_C_DOUBLE * ATAN(1.0_C_DOUBLE)
   dx = 2.0_C_DOUBLE * pi / N
-  xi = 2.5_C_DOUBLE
-  csound = SQRT(2.0_C_DOUBLE) * xi * dx
-  CALL initialize_fft(N, 1.0e-3_C_DOUBLE, xi, csound, 1.0_C_DOUBLE, 0.0_C_DOUBLE)
+  xi_l = 2.5_C_DOUBLE
+  cs_l = SQRT(2.0_C_DOUBLE) * xi_l * dx
+  CALL GET_COMMAND_ARGUMENT(1, outname)
+  CALL GET_COMMAND_ARGUMENT(2, outdir)
+  data_dir = TRIM(outdir)
+  spectra_dir = TRIM(outdir)
+  CALL initialize_fft(N, 1.0e-3_C_DOUBLE, xi_l, cs_l, 1.0_C_DOUBLE, 0.0_C_DOUBLE)
   k = 0
   DO i = 0, 3
     DO j = 0, 3
@@ -37,7 +42,6 @@ PROGRAM driver_check
   END DO
   CALL diagnostics_gpe(0, nsteps, nsteps * 1.0e-3_C_DOUBLE)
   CALL sync_host()
-  CALL GET_COMMAND_ARGUMENT(1, outname)
   OPEN(NEWUNIT=u, FILE=TRIM(outname), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
   WRITE(u) psi, kpsi, egrad, echem, eintr, etot, norm, momx, momy, spec_gp
   Ntt = 32

@@ -2,10 +2,17 @@
 ! the reference's module API so a Fortran driver can call it in place of
 ! mod_fft / gpe / diagnostics:
 !   initialize_fft            <- src/mod_fft.f90:288 (plans) + create_state_variables :122
+!                                (argument-less on the namelist globals, as in
+!                                the reference, or with explicit arguments)
 !   fft(plan_name, direction) <- src/mod_fft.f90:471  (plan 'psi')
 !   gpe_rk4                   <- src/gpe.f90:310
 !   ARGLE(time)               <- src/gpe.f90:5
-!   diagnostics_gpe(spoint,jtime,time) <- src/diagnostics.f90:4
+!   diagnostics_gpe(spoint,jtime,time) <- src/diagnostics.f90:4, with the
+!                                files of :240-259, :530-569 (particles),
+!                                quantglobal_gpe :573 and spectrum_gpe :608
+!   particle_potential        <- src/gpe_particle.f90:3
+!   initialize_particles      <- src/gpe_particle.f90:92 (restart/part files)
+!   gppart_checkpoint(iopoint,jtime,ptime) <- src/gpe_particle.f90:143
 !   io_fields(iopoint,time)   <- src/io_fields.f90:1 (wf files, IOEVERYPROC layout)
 !   restart_gpe(nrestart)     <- src/io_fields.f90:71 (with nsplit regrouping)
 !   clear_fft                 <- src/mod_fft.f90:433
@@ -129,6 +136,30 @@ MODULE vikshobha_c
       TYPE(C_PTR), VALUE :: s
       REAL(C_DOUBLE) :: spec2d(*), spec1d(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_particles_set(h, pos, vel, fext) BIND(C, NAME='vk_gpe_particles_set')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: pos(*), vel(*), fext(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_particles_get(h, pos, vel, frc) BIND(C, NAME='vk_gpe_particles_get')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: pos(*), vel(*), frc(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_part_checkpoint(dir, nrst, iopoint, ptime, np, pos, vel, fext) &
+        BIND(C, NAME='vk_part_checkpoint')
+      IMPORT :: C_INT, C_DOUBLE, C_CHAR
+      CHARACTER(KIND=C_CHAR) :: dir(*)
+      INTEGER(C_INT), VALUE :: nrst, iopoint, np
+      REAL(C_DOUBLE), VALUE :: ptime
+      REAL(C_DOUBLE) :: pos(*), vel(*), fext(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_part_restart(dir, nrst, np, pos, vel, fext) BIND(C, NAME='vk_part_restart')
+      IMPORT :: C_INT, C_DOUBLE, C_CHAR
+      CHARACTER(KIND=C_CHAR) :: dir(*)
+      INTEGER(C_INT), VALUE :: nrst, np
+      REAL(C_DOUBLE) :: pos(*), vel(*), fext(*)
+    END FUNCTION
     FUNCTION vk_last_error() BIND(C, NAME='vk_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR) :: vk_last_error
@@ -160,6 +191,23 @@ MODULE vikshobha_gpu
   INTEGER :: Ntt = 2048
   TYPE(C_PTR) :: vks = C_NULL_PTR
   REAL(C_DOUBLE), ALLOCATABLE :: spec_sts2d(:,:), arr1(:,:)
+  ! the namelist inputs the argument-less initialize_fft reads
+  ! (src/driver.f90:52-78; xi in units of dx, ppotV0 in units of g, ppotd in
+  ! units of xi), the restart number that names the output files, and the
+  ! diagnostic strides of src/diagnostics.f90:430-431
+  INTEGER :: Nx_in = 0, nrst = 0, gsteps = 1, ssteps = 1
+  REAL(C_DOUBLE) :: dt_fix = 0.0_C_DOUBLE, xi = 0.0_C_DOUBLE, csound = 0.0_C_DOUBLE
+  REAL(C_DOUBLE) :: rho_avg = 1.0_C_DOUBLE, disp_factor = 0.0_C_DOUBLE
+  CHARACTER(LEN=256) :: data_dir = 'data', spectra_dir = 'spectra', part_in_dir = 'restart/part'
+  ! particles (mod_gpparticles, src/mod_gpparticles.f90:14-30): (ndim, gp_N_obj)
+  INTEGER :: gp_N_obj = 0
+  REAL(C_DOUBLE) :: ppotV0 = 0.0_C_DOUBLE, ppotd = 0.0_C_DOUBLE, gp_mass_part = 1.0_C_DOUBLE
+  REAL(C_DOUBLE), ALLOCATABLE :: gp_pos_obj(:,:), gp_vel_obj(:,:), gp_f0ext_obj(:,:), gp_frc_obj(:,:)
+  REAL(C_DOUBLE) :: eppot = 0.0_C_DOUBLE, eobj_kin = 0.0_C_DOUBLE, esrep = 0.0_C_DOUBLE, mom_obj(2) = 0.0_C_DOUBLE
+
+  INTERFACE initialize_fft
+    MODULE PROCEDURE initialize_fft_namelist, initialize_fft_args
+  END INTERFACE
 
 CONTAINS
 
@@ -181,17 +229,48 @@ CONTAINS
   END SUBROUTINE
 
   ! set_grid + constants (src/mod_grid.f90:63-101, src/driver.f90:360-376)
-  ! followed by create_state_variables + initialize_fft
-  SUBROUTINE initialize_fft(N, dt_fix, xi, csound, rho_avg, disp_factor)
+  ! followed by create_state_variables + initialize_fft, on the namelist
+  ! globals Nx_in, dt_fix, xi, csound, rho_avg, disp_factor and, for
+  ! gp_N_obj > 0, the particle inputs (ppotV0 = ppotV0*gstrength and
+  ! ppotd = ppotd*xi as src/driver.f90:390-392; create_particles)
+  SUBROUTINE initialize_fft_namelist()
+    CALL vk_check(vk_gpe_params_init(params, INT(Nx_in, C_INT), dt_fix, xi, csound, rho_avg, disp_factor), &
+                  'params_init')
+    IF (gp_N_obj > 0) THEN
+      params%np = INT(gp_N_obj, C_INT)
+      params%V0 = ppotV0 * params%g
+      params%d = ppotd * params%xi
+      params%mass = gp_mass_part
+      params%force_dealias = 1_C_INT
+      ALLOCATE(gp_pos_obj(2, gp_N_obj), gp_vel_obj(2, gp_N_obj), gp_f0ext_obj(2, gp_N_obj), &
+               gp_frc_obj(2, gp_N_obj))
+      gp_pos_obj = 0.0_C_DOUBLE; gp_vel_obj = 0.0_C_DOUBLE
+      gp_f0ext_obj = 0.0_C_DOUBLE; gp_frc_obj = 0.0_C_DOUBLE
+    END IF
+    CALL create_state()
+  END SUBROUTINE
+
+  SUBROUTINE initialize_fft_args(N, dt_in, xi_in, csound_in, rho_avg_in, disp_in)
     INTEGER, INTENT(IN) :: N
-    REAL(C_DOUBLE), INTENT(IN) :: dt_fix, xi, csound, rho_avg, disp_factor
-    CALL vk_check(vk_gpe_params_init(params, INT(N, C_INT), dt_fix, xi, csound, rho_avg, disp_factor), 'params_init')
+    REAL(C_DOUBLE), INTENT(IN) :: dt_in, xi_in, csound_in, rho_avg_in, disp_in
+    Nx_in = N; dt_fix = dt_in; xi = xi_in; csound = csound_in; rho_avg = rho_avg_in; disp_factor = disp_in
+    gp_N_obj = 0
+    CALL initialize_fft_namelist()
+  END SUBROUTINE
+
+  SUBROUTINE create_state()
     CALL vk_check(vk_gpe_create(vkh, params), 'create')
-    Nx = N; Ny = N
-    nshell = vk_nshell(INT(N, C_INT))
+    Nx = Nx_in; Ny = Nx_in
+    nshell = vk_nshell(INT(Nx, C_INT))
     ALLOCATE(psi(Nx, Ny), kpsi(Ny, Nx), spec_gp(0:nshell, 1:6))
     psi = (0.0_C_DOUBLE, 0.0_C_DOUBLE); kpsi = psi
   END SUBROUTINE
+
+  FUNCTION fnrst() RESULT(s)
+    CHARACTER(LEN=16) :: s
+    WRITE(s, '(I8)') nrst
+    s = ADJUSTL(s)
+  END FUNCTION
 
   ! fft(plan_name, direction) for plan 'psi' (src/mod_fft.f90:482-487):
   ! forward: kpsi = FFT(psi); inverse: psi = IFFT(kpsi)/(Nx*Ny)
@@ -234,13 +313,116 @@ CONTAINS
     CALL vk_check(vk_gpe_argle_steps(vkh, 1_C_INT, argle_residual), 'ARGLE')
   END SUBROUTINE
 
+  ! diagnostics_gpe (src/diagnostics.f90:4-571): energies, norm, momentum and
+  ! the six spectra, appended to data/energy1, norm and mom.dat_nrst<nrst>
+  ! (:240-259); with particles the forces and the per-particle and energy
+  ! files (:523-569); quantglobal_gpe every gsteps and spectrum_gpe every
+  ! ssteps (:430-431)
   SUBROUTINE diagnostics_gpe(spoint, jtime, time)
     INTEGER, INTENT(IN) :: spoint, jtime
     REAL(C_DOUBLE), INTENT(IN) :: time
     REAL(C_DOUBLE) :: scal(8), pscal(5)
+    INTEGER :: u, i1
+    CHARACTER(LEN=16) :: nobj
     CALL vk_check(vk_gpe_diagnostics(vkh, scal, spec_gp, pscal), 'diagnostics_gpe')
     egrad = scal(1); echem = scal(2); eintr = scal(3); etot = scal(4)
     norm = scal(5); momx = scal(6); momy = scal(7); n0 = scal(8)
+    OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/energy1.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+    WRITE(u, '(E15.6,E15.6,E15.6,E15.6,E15.6)') time, egrad, echem, eintr, etot
+    CLOSE(u)
+    OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/norm.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+    WRITE(u, '(E15.6,E15.6)') time, norm
+    CLOSE(u)
+    OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/mom.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+    WRITE(u, '(E15.6,E15.6,E15.6)') time, momx, momy
+    CLOSE(u)
+    IF (MOD(jtime, gsteps) == 0) CALL quantglobal_gpe(jtime, time)
+    IF (MOD(jtime, ssteps) == 0) CALL spectrum_gpe(spoint, time)
+    IF (params%np > 0) THEN
+      eppot = pscal(1); eobj_kin = pscal(2); esrep = pscal(3); mom_obj = pscal(4:5)
+      CALL vk_check(vk_gpe_particles_get(vkh, gp_pos_obj, gp_vel_obj, gp_frc_obj), 'diagnostics_gpe')
+      DO i1 = 1, gp_N_obj
+        WRITE(nobj, '(I8)') i1
+        nobj = ADJUSTL(nobj)
+        OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/gpobj_pos_'//TRIM(nobj)//'.dat_nrst'//TRIM(fnrst()), &
+             POSITION='APPEND')
+        WRITE(u, '(E15.6,E15.6,E15.6)') time, gp_pos_obj(1, i1), gp_pos_obj(2, i1)
+        CLOSE(u)
+        OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/gpobj_vel_'//TRIM(nobj)//'.dat_nrst'//TRIM(fnrst()), &
+             POSITION='APPEND')
+        WRITE(u, '(E15.6,E15.6,E15.6)') time, gp_vel_obj(1, i1), gp_vel_obj(2, i1)
+        CLOSE(u)
+        OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/gpobj_frc_'//TRIM(nobj)//'.dat_nrst'//TRIM(fnrst()), &
+             POSITION='APPEND')
+        WRITE(u, '(E15.6,E15.6,E15.6)') time, gp_frc_obj(1, i1), gp_frc_obj(2, i1)
+        CLOSE(u)
+      END DO
+      OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/egpobj.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+      WRITE(u, '(E15.6,E15.6,E15.6,E15.6)') time, eppot, eobj_kin, esrep
+      CLOSE(u)
+      OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/momgpobj.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+      WRITE(u, '(E15.6,E15.6,E15.6)') time, mom_obj(1), mom_obj(2)
+      CLOSE(u)
+    END IF
+  END SUBROUTINE
+
+  ! quantglobal_gpe (src/diagnostics.f90:573-606): spectral totals and the
+  ! condensate occupation n0
+  SUBROUTINE quantglobal_gpe(jtime, time)
+    INTEGER, INTENT(IN) :: jtime
+    REAL(C_DOUBLE), INTENT(IN) :: time
+    INTEGER :: u
+    OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/energy.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+    WRITE(u, '(E15.6,E15.6,E15.6,E15.6,E15.6,E15.6)') time, SUM(spec_gp(1:nshell, 1)), &
+        SUM(spec_gp(1:nshell, 2)), SUM(spec_gp(1:nshell, 3)), SUM(spec_gp(1:nshell, 4)), SUM(spec_gp(0:nshell, 5))
+    CLOSE(u)
+    OPEN(NEWUNIT=u, FILE=TRIM(data_dir)//'/n0.dat_nrst'//TRIM(fnrst()), POSITION='APPEND')
+    WRITE(u, '(E15.6,E15.6)') time, spec_gp(0, 6)
+    CLOSE(u)
+  END SUBROUTINE
+
+  ! spectrum_gpe (src/diagnostics.f90:608-634): spectra/spec<spoint>.dat,
+  ! shells 1..nshell
+  SUBROUTINE spectrum_gpe(spoint, time)
+    INTEGER, INTENT(IN) :: spoint
+    REAL(C_DOUBLE), INTENT(IN) :: time
+    INTEGER :: u, i1
+    CHARACTER(LEN=16) :: fnum
+    WRITE(fnum, '(I8)') spoint
+    fnum = ADJUSTL(fnum)
+    OPEN(NEWUNIT=u, FILE=TRIM(spectra_dir)//'/spec'//TRIM(fnum)//'.dat', STATUS='UNKNOWN')
+    DO i1 = 1, nshell
+      WRITE(u, '(E15.6,E15.6,E15.6,E15.6,E15.6,E15.6,E15.6)') DBLE(i1), spec_gp(i1, 1), spec_gp(i1, 2), &
+          spec_gp(i1, 3), spec_gp(i1, 4), spec_gp(i1, 5), spec_gp(i1, 6)
+    END DO
+    CLOSE(u)
+  END SUBROUTINE
+
+  ! particle_potential (src/gpe_particle.f90:3-90): the potential of one
+  ! particle at the origin, transformed and dealiased on the device (the
+  ! particle state is handed over with it)
+  SUBROUTINE particle_potential()
+    CALL vk_check(vk_gpe_particles_set(vkh, gp_pos_obj, gp_vel_obj, gp_f0ext_obj), 'particle_potential')
+  END SUBROUTINE
+
+  ! initialize_particles (src/gpe_particle.f90:92-141): positions,
+  ! velocities and external forces from
+  ! <part_in_dir>/gppos|gpvel|gpf0ext.dat_nrst<nrst>, then to the device
+  SUBROUTINE initialize_particles()
+    CALL vk_check(vk_part_restart(TRIM(part_in_dir)//C_NULL_CHAR, INT(nrst, C_INT), INT(gp_N_obj, C_INT), &
+                                  gp_pos_obj, gp_vel_obj, gp_f0ext_obj), 'initialize_particles')
+    CALL vk_check(vk_gpe_particles_set(vkh, gp_pos_obj, gp_vel_obj, gp_f0ext_obj), 'initialize_particles')
+  END SUBROUTINE
+
+  ! gppart_checkpoint (src/gpe_particle.f90:143-184): appends the current
+  ! positions, velocities and external forces to
+  ! <data_dir>/gppos|gpvel|gpf0ext_ckp.dat_nrst<nrst>
+  SUBROUTINE gppart_checkpoint(iopoint, jtime, ptime)
+    INTEGER, INTENT(IN) :: iopoint, jtime
+    REAL(C_DOUBLE), INTENT(IN) :: ptime
+    CALL vk_check(vk_gpe_particles_get(vkh, gp_pos_obj, gp_vel_obj, gp_frc_obj), 'gppart_checkpoint')
+    CALL vk_check(vk_part_checkpoint(TRIM(data_dir)//C_NULL_CHAR, INT(nrst, C_INT), INT(iopoint, C_INT), ptime, &
+                                     INT(gp_N_obj, C_INT), gp_pos_obj, gp_vel_obj, gp_f0ext_obj), 'gppart_checkpoint')
   END SUBROUTINE
 
   ! io_fields (src/io_fields.f90:41-50): psi to wf<iopoint>p<r>.dat
@@ -298,6 +480,7 @@ CONTAINS
     IF (C_ASSOCIATED(vkh)) CALL vk_gpe_destroy(vkh)
     vkh = C_NULL_PTR
     IF (ALLOCATED(psi)) DEALLOCATE(psi, kpsi, spec_gp)
+    IF (ALLOCATED(gp_pos_obj)) DEALLOCATE(gp_pos_obj, gp_vel_obj, gp_f0ext_obj, gp_frc_obj)
   END SUBROUTINE
 
 END MODULE vikshobha_gpu

@@ -358,8 +358,44 @@ def test_particles_rk4_vs_oracle(G, oracle_mod, ppotd):
     d = g.diagnostics_gpe()
     scal, spec, pscal = st.diagnostics()
     say("  frc gpu", d["frc"].ravel(), "orc", st.part(3).ravel())
+    say("  pscal gpu", [d["eppot"], d["eobj_kin"], d["esrep"], *d["momobj"]], "orc", list(pscal))
     assert rel_l2(d["frc"], st.part(3)) < 1e-10
     assert abs(d["eppot"] - pscal[0]) <= 1e-11 * abs(pscal[0])
+    # particle kinetic energy, short-range energy and momentum
+    # (src/diagnostics.f90:210-239)
+    assert abs(d["eobj_kin"] - pscal[1]) <= 1e-10 * abs(pscal[1])
+    assert abs(d["esrep"] - pscal[2]) <= 1e-11 * abs(pscal[2])
+    assert np.abs(d["momobj"] - pscal[3:5]).max() <= 1e-10 * np.abs(pscal[3:5]).max()
+    assert pscal[1] > 0 and pscal[2] > 0
+    g.close()
+
+
+def test_particles_wide_potential_full_window(G, oracle_mod):
+    """Real-space potential path with a wide potential (d = 0.3 at 256^2):
+    the periodic coordinate saturates near the half box, so the 10 d window
+    would cut V off at ~1e-14 V0; the window then spans the whole period."""
+    N, dt, steps, npart = 256, 1e-3, 3, 3
+    O = oracle_mod
+    ppotd = 0.3 / (2.5 * 2 * np.pi / N)
+    say("case particles wide potential N=%d d=0.3 (ppotd=%.3f)" % (N, ppotd))
+    psi0 = _ic(G, N)
+    pos = np.array([[1.2, 0.4], [3.3, 2.9], [5.1, 5.6]])
+    vel = np.array([[0.05, -0.02], [0.0, 0.03], [-0.04, 0.01]])
+    kw = dict(gp_N_obj=npart, ppotV0=1.0, ppotd=ppotd, gp_mass_part=50.0)
+    g = G.GPE(G.params_from_namelist(N, dt, 2.5, **kw))
+    g.set_psi(psi0, sync=True)
+    g.initialize_particles(pos, vel)
+    g.gpe_rk4(steps)
+    st = O.State(O.make_params(N, dt, 2.5, np_=npart, V0_in=1.0, d_in=ppotd, mass=50.0))
+    st.set_psi(psi0, sync=True)
+    st.part(0)[...] = pos
+    st.part(1)[...] = vel
+    st.particle_potential()
+    st.rk4(steps)
+    gp, gv, _ = g.particles()
+    e = rel_l2(g.psi, st.psi)
+    say("  rel-L2 psi %.3e pos %.3e vel %.3e" % (e, rel_l2(gp, st.part(0)), rel_l2(gv, st.part(1))))
+    assert e < TOL and rel_l2(gp, st.part(0)) < TOL and rel_l2(gv, st.part(1)) < 1e-10
     g.close()
 
 
