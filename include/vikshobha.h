@@ -126,6 +126,19 @@ int vk_gpe_set_advection(vk_gpe* h, int taylor_green, double vnx, double vny);
  * (Ny,Nx) transposed, unnormalised; dir = +1: inverse times 1/(Nx*Ny). */
 int vk_fft2d(vk_gpe* h, int dir, const double* in, double* out);
 
+/* Allocation-free variant of vk_fft2d (no counterpart in the reference; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  in and out are distinct device
+ * buffers of N*N complex doubles in vk_fft2d's layouts; the intermediate
+ * lives in a fixed library scratch of 1024^2 complex doubles (16 MiB per
+ * GPU, held until the process ends), so N <= 1024 (larger N:
+ * VK_ERR_UNSUPPORTED).  No hipMalloc, hipFree or other allocation happens
+ * in the call.  Stream-ordered on the handle's stream (vk_gpe_stream): in
+ * must be complete when the call is made, out is complete once that stream
+ * has drained.  All calls on one GPU share the scratch: calls that overlap
+ * on one GPU (other handles' streams, other host threads) race. */
+int vk_fft2d_noalloc(vk_gpe* h, int dir, const double* in, double* out);
+
 /* Padé-core vortex product initial condition (no reference counterpart:
  * the reference reads its initial state from restart files).
  * nv vortices at (xs, ys) with charges qs (+-1), healing length xi,

@@ -37,6 +37,17 @@ namespace {
 thread_local std::string g_err;
 }
 
+// Scratch of the allocation-free entry point vk_fft2d_noalloc: one
+// half-spectral field (the row pass output the column pass reads) of the
+// largest size that entry point accepts, VK_NOALLOC_MAX_N^2 complex doubles
+// = 16 MiB.  Every GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes this name to a kernel: its device
+// address on each device is looked up once, in vk_gpe_create with that
+// device current (scratch_setup), and kept in g_scratch_ptr.
+#define VK_NOALLOC_MAX_N 1024
+#define VK_MAX_DEVICES 64
+static __device__ __align__(256) double2 scratch[(size_t)VK_NOALLOC_MAX_N * VK_NOALLOC_MAX_N];
+
 // shared with the other translation units of the library (vk_last_error)
 int vk_set_error(int code, const std::string& msg)
 {
@@ -71,6 +82,24 @@ int shell_of(int kx, int ky)
 }
 
 bool masked_host(int kx, int ky, int N) { return 9LL * ((long long)kx * kx + (long long)ky * ky) >= (long long)N * N; }
+
+std::mutex g_scratch_mu;
+cplx* g_scratch_ptr[VK_MAX_DEVICES] = {};
+
+// device address of `scratch` on device dev, which must be current; looked
+// up on the first handle created on that device and kept
+int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= VK_MAX_DEVICES) return fail(VK_ERR_UNSUPPORTED, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(VK_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<cplx*>(ptr);
+    return 0;
+}
 int ksg_host(int i, int N) { return i < N / 2 ? i : i - N; }
 
 }  // namespace
@@ -89,6 +118,7 @@ struct vk_gpe {
     cplx *K0 = nullptr, *K1 = nullptr, *K2 = nullptr, *Hpsi = nullptr, *Hrho = nullptr, *HN = nullptr;
     cplx* Xs = nullptr;   // x-space psi rows written by P1 and read by P3 (xpsi)
     int xpsi = 0;
+    int dev = 0;        // device the handle was created on
     int ncu = 0;        // compute units of the device
     int p1pf = 0;       // RK4 P1 as the persistent prefetching kernel (4096^2)
     // advective ARGLE (Taylor-Green and/or counterflow); Hdy = IFFT_y(i ky kpsi)
@@ -794,6 +824,8 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return cleanup(fail(VK_ERR_HIP, "device attribute query failed"));
+        h->dev = dev;
+        if (int rc = scratch_setup(dev)) return cleanup(rc);
         h->p1pf = p1_prefetch_ok(h);
     }
     if (h->xpsi && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
@@ -1152,6 +1184,36 @@ int vk_fft2d(vk_gpe* h, int dir, const double* in, double* out)
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(out, h->X[0], fb, hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+// The transforms of vk_fft2d between caller-owned device buffers, with the
+// half-spectral intermediate in the static scratch instead of library
+// memory: no allocation of any kind.  Stream-ordered on the handle's stream
+// (returns once the launches are queued).
+int vk_fft2d_noalloc(vk_gpe* h, int dir, const double* in, double* out)
+{
+    if (!h || !in || !out || (dir != -1 && dir != 1) || (const void*)in == (const void*)out)
+        return fail(VK_ERR_ARG, "fft2d_noalloc: bad arguments (distinct device buffers, dir = -1 or +1)");
+    if (h->N > VK_NOALLOC_MAX_N)
+        return fail(VK_ERR_UNSUPPORTED, "fft2d_noalloc: N = " + std::to_string(h->N) + " exceeds the " +
+                                            std::to_string(VK_NOALLOC_MAX_N) + "^2 scratch");
+    for (const void* q : {(const void*)in, (const void*)out}) {
+        hipPointerAttribute_t a{};
+        if (hipPointerGetAttributes(&a, q) != hipSuccess || a.type != hipMemoryTypeDevice) {
+            (void)hipGetLastError();
+            return fail(VK_ERR_ARG, "fft2d_noalloc: in and out must be device memory");
+        }
+    }
+    cplx* H = g_scratch_ptr[h->dev];
+    if (!H) return fail(VK_ERR_STATE, "fft2d_noalloc: scratch not set up on this device");
+    const cplx* ci = reinterpret_cast<const cplx*>(in);
+    cplx* co = reinterpret_cast<cplx*>(out);
+    int rc;
+    if (dir < 0) rc = fwd2d(h, ci, H, co, 0, 0);
+    else if (!(rc = col_inv(h, ci, H, 0, 0))) rc = row_inv(h, H, co, 0, 0, scale_of(h));
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
