@@ -121,6 +121,42 @@ int vk_gpe_set_dt(vk_gpe* h, double dt);
  * RK4 is unaffected, as in the reference. */
 int vk_gpe_set_advection(vk_gpe* h, int taylor_green, double vnx, double vny);
 
+/* Advective ARGLE with a caller-supplied velocity u(x,y) in place of the
+ * Taylor-Green field of the GPE_ADVECTION slot (src/gpe.f90:114-130; same
+ * terms -i u.grad psi - |u|^2 psi/(4 alpha), the counterflow of
+ * vk_gpe_set_advection still added): ux, uy are (Nx,Ny) real arrays, host
+ * or device; both NULL turns the field off.
+ * vk_gpe_set_vortex_advection sets the imposed velocity of a vortex
+ * configuration, the current of its Padé product ansatz
+ * u = 2 alpha rho0 grad(theta0) (smooth, zero in the cores): ARGLE then
+ * relaxes the cores in place instead of letting vortices drift and
+ * annihilate.  vk_vortex_velocity_host computes the same field on the CPU. */
+int vk_gpe_set_advection_field(vk_gpe* h, const double* ux, const double* uy);
+int vk_gpe_set_vortex_advection(vk_gpe* h, int nv, const double* xs, const double* ys, const int* qs, int nimg);
+int vk_vortex_velocity_host(int N, int nv, const double* xs, const double* ys, const int* qs, double xi,
+                            double alpha, int nimg, double* ux, double* uy);
+
+/* SGLE, the GPE_SGLE build of ARGLE(time) (src/gpe.f90:149-217 noise and
+ * update, :226-266 finiteT_adhoc_chempot; namelist gpfiniteT
+ * oneybeta, nuN, chempot0, src/driver.f90:59,376).  With on = 1 every ARGLE
+ * step adds sqrt(dt) P_G[FFT(noisesigma xi/sqrt(dvol))], noisesigma =
+ * sqrt(oneybeta/(2 alpha)), and first relaxes the chemical potential,
+ * omega -= dt (nuN/L^2)(norm - L^2), starting from chempot0; ktrunc > 0
+ * zeroes kpsi beyond |k| = ktrunc (GPE_SELFTRUNC).  xi is a field of
+ * complex standard normals: vk_gpe_argle_steps / _until draw it from a
+ * counter-based generator (Philox4x32-10 keyed by seed, draw n of the
+ * handle's sequence, Box-Muller); the reference's Fortran RANDOM_NUMBER
+ * stream cannot be reproduced, only its distribution.
+ * vk_gpe_argle_step_noise runs one step with the caller's xi
+ * (noise_xy: (Nx,Ny) complex, host or device).  vk_sgle_noise writes draw
+ * `draw` of the generator (unscaled).  vk_gpe_get_chempot returns the
+ * current omega. */
+int vk_gpe_set_sgle(vk_gpe* h, int on, double oneybeta, double nuN, double chempot0, double ktrunc,
+                    unsigned long long seed);
+int vk_gpe_argle_step_noise(vk_gpe* h, const double* noise_xy, double* residual);
+int vk_sgle_noise(vk_gpe* h, unsigned long long draw, double* out_xy);
+int vk_gpe_get_chempot(vk_gpe* h, double* omega);
+
 /* Conformance hook mirroring fft(plan_name, direction)
  * (src/mod_fft.f90:471-542): dir = -1: in = x-space (Nx,Ny) -> out = k-space
  * (Ny,Nx) transposed, unnormalised; dir = +1: inverse times 1/(Nx*Ny). */

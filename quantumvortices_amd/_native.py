@@ -56,6 +56,15 @@ SIGNATURES = {
     "vk_gpe_set_advection": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "vk_gpe_set_obstacles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]),
     "vk_fft2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
+    "vk_gpe_set_advection_field": (ctypes.c_int, [ctypes.c_void_p, dp, dp]),
+    "vk_gpe_set_vortex_advection": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, ip, ctypes.c_int]),
+    "vk_vortex_velocity_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, dp, dp]),
+    "vk_gpe_set_sgle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
+    "vk_gpe_argle_step_noise": (ctypes.c_int, [ctypes.c_void_p, dp, dp]),
+    "vk_sgle_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, dp]),
+    "vk_gpe_get_chempot": (ctypes.c_int, [ctypes.c_void_p, dp]),
     "vk_fft2d_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
     "vk_ic_vortices_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_double, ctypes.c_int, dp]),
     "vk_gpe_set_ic_vortices": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, ip, ctypes.c_int, ctypes.c_int]),

@@ -97,4 +97,34 @@ __device__ __forceinline__ size_t hoff(int p, int y, int c)
     return ((size_t)p * C::N + y) * C::TC + c;
 }
 
+// Raw buffer access.  A buffer load whose byte offset is at or beyond the
+// resource's size returns zero, and a buffer store there is dropped.  The
+// passes issue each load unconditionally, with kOOB as the offset of the
+// points they do not need, and select afterwards: a load inside a branch
+// whose result is used in the same branch makes the compiler wait for that
+// load before the next one is issued, which serialises the whole load
+// phase.  Fields are at most 4096^2 complex doubles (256 MiB), so byte
+// offsets fit in 32 bits and kOOB is beyond every field.
+typedef __amdgpu_buffer_rsrc_t brsrc;
+constexpr unsigned kOOB = 0x80000000u;
+__device__ __forceinline__ brsrc mkbuf(const void* p, int N)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, N * N * 16, 0x00020000);
+}
+__device__ __forceinline__ cplx bld(brsrc r, unsigned off)
+{
+    const auto w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    cplx c;
+    __builtin_memcpy(&c, &w, sizeof(c));
+    return c;
+}
+__device__ __forceinline__ void bst(brsrc r, unsigned off, cplx c)
+{
+    __attribute__((ext_vector_type(4))) unsigned w;
+    __builtin_memcpy(&w, &c, sizeof(c));
+    __builtin_amdgcn_raw_buffer_store_b128(w, r, off, 0, 0);
+}
+// byte offset of element i, or kOOB
+__device__ __forceinline__ unsigned boff(size_t i, bool use) { return use ? (unsigned)(i * 16) : kOOB; }
+
 }  // namespace vk

@@ -67,7 +67,12 @@ __global__ void __launch_bounds__(256) k_ens_p4(const EnsReal* __restrict__ R, c
     a.cstep = stage == 3 ? e.dt : e.dt * 0.5;
     a.r1 = (e.dt * oney6) / (e.dt * 0.5);
     a.r3 = (e.dt * oney6 * 2.0) / e.dt;
-    VK_ENS_SWITCH(e.N, (p4_body<C>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a)))
+    switch (stage) {
+    case 1: VK_ENS_SWITCH(e.N, (p4_body<C, 1>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a))) break;
+    case 2: VK_ENS_SWITCH(e.N, (p4_body<C, 2>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a))) break;
+    case 3: VK_ENS_SWITCH(e.N, (p4_body<C, 3>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a))) break;
+    default: VK_ENS_SWITCH(e.N, (p4_body<C, 4>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, a))) break;
+    }
 }
 
 }  // namespace vk

@@ -121,10 +121,22 @@ struct vk_gpe {
     int dev = 0;        // device the handle was created on
     int ncu = 0;        // compute units of the device
     int p1pf = 0;       // RK4 P1 as the persistent prefetching kernel (4096^2)
+    int tr1_p1 = 0, tr1_p3 = 0;   // RK4 P1 / P3 (psi-row path) with one row per workgroup
     // advective ARGLE (Taylor-Green and/or counterflow); Hdy = IFFT_y(i ky kpsi)
     int adv_tg = 0;
     double adv_vx = 0.0, adv_vy = 0.0;
     cplx* Hdy = nullptr;
+    // caller-supplied advection velocity (replaces the Taylor-Green field)
+    double *Ux = nullptr, *Uy = nullptr;
+    int adv_field = 0;
+    // SGLE (GPE_SGLE): noise field (x-space, then k-space), the chemical
+    // potential on the device, norm partials, the draw counter of the
+    // generator
+    int sgle = 0;
+    double sg_beta = 0.0, sg_nuN = 0.0, sg_ktrunc = 0.0;
+    unsigned long long sg_seed = 0, sg_draw = 0;
+    cplx *Xn = nullptr, *Kn = nullptr;
+    double *omega_dev = nullptr, *npart = nullptr;
     cplx *HV = nullptr, *KV = nullptr, *Krho = nullptr, *Vhat = nullptr, *EX = nullptr, *EY = nullptr;
     double *fpart = nullptr, *pst = nullptr, *part = nullptr, *dsum = nullptr;
     unsigned char* phit = nullptr;   // real-space path: window flags per row block and particle
@@ -159,6 +171,10 @@ namespace {
 size_t f2(const vk_gpe* h) { return (size_t)h->N * h->N; }
 
 template <int N> using CfgN = vk::Cfg<N>;
+// The same panel layout with one row per row-pass workgroup (256 threads)
+// instead of TR rows: more, smaller workgroups per CU at the cost of half
+// as long row pieces of the panels.
+template <class C> using CfgTR1 = vk::CfgT<C::N, C::S, C::TC, 1, C::SPLIT>;
 
 template <class F>
 int dispatch(int N, F&& f)
@@ -284,9 +300,10 @@ void p2_launch(vk_gpe* h, A... args)
 int p1_prefetch_ok(const vk_gpe* h)
 {
     if (!VK_P1_PREFETCH || h->N != 4096 || !h->xpsi) return 0;
-    // VK_NO_P1_PREFETCH=1 selects the plain P1 (tests compare the two)
-    const char* off = std::getenv("VK_NO_P1_PREFETCH");
-    if (off && off[0] == '1') return 0;
+    // the plain P1, whose loads are all in flight at once, is faster; the
+    // prefetching kernel runs when VK_P1_PREFETCH=1 (tests compare the two)
+    const char* on = std::getenv("VK_P1_PREFETCH");
+    if (!on || on[0] != '1') return 0;
     using C = CfgN<4096>;
     static_assert(vk::p1p_lds<C>() <= 160 * 1024, "prefetch buffer exceeds LDS");
     const int g8 = h->ncu / 8, r8 = (C::N / C::TR) / 8;
@@ -494,7 +511,7 @@ int particle_forces(vk_gpe* h, const cplx* Krho, int nrows, int half, int stage)
 // particle stage update from the real-space block partials written by P3
 // The update depends on P3 (force partials) and is needed by the next
 // P3 (positions); it runs on a side stream while P4, P1, P2 proceed.
-int particle_step_rs(vk_gpe* h, int nblk, int stage)
+int particle_step_rs(vk_gpe* h, int nblk, int nw, int stage)
 {
     const bool side = !h->timing && h->pstream;
     hipStream_t st = side ? h->pstream : h->stream;
@@ -505,8 +522,6 @@ int particle_step_rs(vk_gpe* h, int nblk, int stage)
     ev_begin(h);
     const double dx = 2.0 * (4.0 * std::atan(1.0)) / h->N;
     const double coef = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
-    int nw = 1;
-    dispatch(h->N, [&](auto n) { nw = (CfgN<decltype(n)::value>::ROWT + 63) / 64; return 0; });
     hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, st, (const double*)h->fpart,
                        (const unsigned char*)h->phit, nblk, nw, h->p.np, coef, h->pst + 28 * h->p.np + 8);
     hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, st, (const double*)h->fpart, (const cplx*)h->EX,
@@ -547,6 +562,9 @@ int rk4_step(vk_gpe* h)
             if (parts && !rs) particle_kv(h, Kp);
             if (h->p1pf)
                 p1p_launch<C>(h, sc);
+            else if (h->xpsi && h->tr1_p1)
+                row_launch<CfgTR1<C>>(h, PK_P1, vk::k_p1x<CfgTR1<C>>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw,
+                                      h->geo, 1, sc, h->Xs);
             else if (h->xpsi)
                 row_launch<C>(h, PK_P1, vk::k_p1x<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 1, sc,
                               h->Xs);
@@ -556,7 +574,12 @@ int rk4_step(vk_gpe* h)
                           (const cplx*)h->tw, h->geo, (parts && !rs) ? 3 : 0);
             if (parts && !rs) particle_forces(h, h->Krho, Kp, 1, stage);
             if (rs) join_particles(h);   // positions from the previous stage
-            if (h->xpsi) {
+            if (h->xpsi && h->tr1_p3) {
+                using C1 = CfgTR1<C>;
+                auto k3 = !parts ? vk::k_p3x<C1, 0> : (rs ? vk::k_p3x<C1, 6> : vk::k_p3x<C1, 1>);
+                row_launch_x<C1>(h, PK_P3, p3_extra<C1>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV,
+                                 (const cplx*)h->Xs, h->HN, (const cplx*)h->tw, h->geo, 1, h->p.g * sc, sc, 1.0, pa);
+            } else if (h->xpsi) {
                 auto k3 = !parts ? vk::k_p3x<C, 0> : (rs ? vk::k_p3x<C, 6> : vk::k_p3x<C, 1>);
                 row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV,
                                 (const cplx*)h->Xs, h->HN, (const cplx*)h->tw, h->geo, 1, h->p.g * sc, sc, 1.0, pa);
@@ -565,13 +588,18 @@ int rk4_step(vk_gpe* h)
                 row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV, (const cplx*)h->Hpsi,
                               h->HN, (const cplx*)h->tw, h->geo, 1, 1, h->p.g * sc, sc, sc, 1.0, pa);
             }
-            if (rs) particle_step_rs(h, C::N / C::TR, stage);
+            if (rs) {
+                if (h->xpsi && h->tr1_p3) particle_step_rs(h, C::N, (CfgTR1<C>::ROWT + 63) / 64, stage);
+                else particle_step_rs(h, C::N / C::TR, (C::ROWT + 63) / 64, stage);
+            }
             a.stage = stage;
             a.wacc = (stage == 1 || stage == 4) ? dt * oney6 : dt * oney6 * 2.0;
             a.cstep = stage == 3 ? dt : dt * 0.5;
             a.r1 = (dt * oney6) / (dt * 0.5);
             a.r3 = (dt * oney6 * 2.0) / dt;
-            col_launch<C>(h, PK_P4, h->geo.Pp + h->geo.Pn, vk::k_p4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
+            auto k4 = stage == 1 ? vk::k_p4<C, 1> : stage == 2 ? vk::k_p4<C, 2> : stage == 3 ? vk::k_p4<C, 3>
+                                                                                        : vk::k_p4<C, 4>;
+            col_launch<C>(h, PK_P4, h->geo.Pp + h->geo.Pn, k4, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
                           h->K2, (const cplx*)h->tw, h->geo, a);
         }
         if (rs) join_particles(h);
@@ -580,8 +608,35 @@ int rk4_step(vk_gpe* h)
     });
 }
 
-int argle_step(vk_gpe* h, int want_res)
+// SGLE part of one ARGLE step, before the passes: the noise field of this
+// step (noise = caller's standard normals, or NULL for the generator's next
+// draw) scaled by noisesigma/sqrt(dvol) (src/gpe.f90:150-164) and
+// transformed to k-space, and the chemical potential of this step from the
+// norm of kpsi at the step's start (finiteT_adhoc_chempot, :226-266).
+constexpr int kNormBlocks = 1024;
+int sgle_prepare(vk_gpe* h, const double* noise)
 {
+    const double pi = 4.0 * std::atan(1.0);
+    const double L = 2.0 * pi, dx = L / h->N, dvol = dx * dx;
+    const double sigma = std::sqrt((1.0 / (2.0 * h->p.alpha)) * h->sg_beta);
+    if (noise) HIPCHK(hipMemcpyAsync(h->Xn, noise, f2(h) * sizeof(cplx), hipMemcpyDefault, h->stream));
+    hipLaunchKernelGGL(vk::k_sgle_noise, dim3(1024), dim3(256), 0, h->stream, h->Xn, h->N, h->sg_draw, h->sg_seed,
+                       1.0 / std::sqrt(dvol), sigma, noise ? 0 : 1);
+    if (!noise) h->sg_draw++;
+    int rc = fwd2d(h, h->Xn, h->Hd, h->Kn, 0, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(vk::k_knorm, dim3(kNormBlocks), dim3(256), 0, h->stream, (const cplx*)h->K0, f2(h), h->npart);
+    hipLaunchKernelGGL(vk::k_chempot, dim3(1), dim3(256), 0, h->stream, (const double*)h->npart, kNormBlocks,
+                       h->omega_dev, dvol / ((double)h->N * (double)h->N), h->p.dt, h->sg_nuN, L * L);
+    return 0;
+}
+
+int argle_step(vk_gpe* h, int want_res, const double* noise = nullptr)
+{
+    if (h->sgle) {
+        int rc = sgle_prepare(h, noise);
+        if (rc) return rc;
+    }
     return dispatch(h->N, [&](auto n) {
         using C = CfgN<decltype(n)::value>;
         const double sc = scale_of(h);
@@ -592,7 +647,7 @@ int argle_step(vk_gpe* h, int want_res)
             particle_kv(h, C::TC * h->geo.Pp);
             h->kv_valid = 1;
         }
-        const bool adv = h->adv_tg || h->adv_vx != 0.0 || h->adv_vy != 0.0;
+        const bool adv = h->adv_tg || h->adv_field || h->adv_vx != 0.0 || h->adv_vy != 0.0;
         if (adv) col_launch<C>(h, PK_OTHER, h->NP, vk::k_col_inv<C>, (const cplx*)h->K0, h->Hdy, (const cplx*)h->tw,
                                h->geo, 0, 2);   // i ky kpsi -> Hdy (all columns)
         if (h->xpsi)
@@ -603,7 +658,9 @@ int argle_step(vk_gpe* h, int want_res)
         p2_launch<C>(h, h->Hrho, h->HV, (const cplx*)h->KV, h->Krho,
                       (const cplx*)h->tw, h->geo, (parts && !rs) ? 1 : 0);
         if (adv) {
-            const vk::AdvArgs aa{h->adv_tg, h->adv_vx, h->adv_vy, 0.25 / h->p.alpha, sc};
+            const vk::AdvArgs aa{h->adv_tg, h->adv_vx, h->adv_vy, 0.25 / h->p.alpha, sc,
+                                 h->adv_field ? (const double*)h->Ux : nullptr,
+                                 h->adv_field ? (const double*)h->Uy : nullptr};
             const cplx* xs = h->xpsi ? (const cplx*)h->Xs : nullptr;
             auto launch = [&](auto k3) {
                 row_launch_x<C>(h, PK_P3, p3_extra<C>(h), k3, (const cplx*)h->Hrho, (const cplx*)h->HV,
@@ -622,6 +679,12 @@ int argle_step(vk_gpe* h, int want_res)
                           (const cplx*)h->tw, h->geo, 0, 0, -h->p.g * sc, -sc, sc, -1.0, pa);
         }
         vk::ArgleArgs a{h->p.alpha, h->p.omega, h->p.dt, want_res};
+        if (h->sgle) {
+            a.Kn = h->Kn;
+            a.omega_dev = h->omega_dev;
+            a.sdt = std::sqrt(h->p.dt);
+            a.ktrunc2 = h->sg_ktrunc > 0.0 ? h->sg_ktrunc * h->sg_ktrunc : 0.0;
+        }
         col_launch<C>(h, PK_ARG4, h->NP, vk::k_a4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, (const cplx*)h->tw, a,
                       h->part);
         if (want_res)
@@ -827,6 +890,10 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         h->dev = dev;
         if (int rc = scratch_setup(dev)) return cleanup(rc);
         h->p1pf = p1_prefetch_ok(h);
+        const char* t1 = std::getenv("VK_P1_TR1");
+        const char* t3 = std::getenv("VK_P3_TR1");
+        h->tr1_p1 = t1 && t1[0] == '1';
+        h->tr1_p3 = t3 && t3[0] == '1';
     }
     if (h->xpsi && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
     if (hipMalloc(&h->tw, sizeof(cplx) * N) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc tw"));
@@ -862,7 +929,7 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         size_t rows = 0, nw = 0;
         dispatch(N, [&](auto n) {
             using C = CfgN<decltype(n)::value>;
-            rows = C::N / C::TR;
+            rows = C::N;                      // one row per block at most
             nw = (C::ROWT + 63) / 64;
             return 0;
         });
@@ -898,8 +965,9 @@ void vk_gpe_destroy(vk_gpe* h)
 {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
-    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd, h->Xs, h->Hdy};
+    cplx* fields[] = {h->K0, h->K1, h->K2, h->Hpsi, h->Hrho, h->HN, h->tw, h->Hd, h->Xs, h->Hdy, h->Xn, h->Kn};
     for (auto f : fields) if (f) hipFree(f);
+    for (double* f : {h->Ux, h->Uy, h->omega_dev, h->npart}) if (f) hipFree(f);
     if (h->np_cap > 0) {
         cplx* pf[] = {h->HV, h->KV, h->Krho, h->Vhat, h->EX, h->EY};
         for (auto f : pf) if (f) hipFree(f);
@@ -924,6 +992,9 @@ void vk_gpe_destroy(vk_gpe* h)
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
+
+static int sgle_sync_omega(vk_gpe* h);
+static int ensure_hdy(vk_gpe* h);
 
 static int need_x0(vk_gpe* h)
 {
@@ -1017,6 +1088,7 @@ int vk_gpe_argle_steps(vk_gpe* h, int nsteps, double* residual)
         if ((rc = argle_step(h, residual && i == nsteps - 1))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipGetLastError());
+    if ((rc = sgle_sync_omega(h))) return rc;
     if (residual && nsteps > 0) {
         double r[2];
         HIPCHK(hipMemcpy(r, h->dsum, sizeof(r), hipMemcpyDeviceToHost));
@@ -1044,6 +1116,7 @@ int vk_gpe_argle_until(vk_gpe* h, double tol, int max_steps, int check_every, in
         if (!(res > tol)) break;   // converged (or NaN: stop and report)
     }
     HIPCHK(hipGetLastError());
+    if ((rc = sgle_sync_omega(h))) return rc;
     if (steps_out) *steps_out = done;
     if (residual) *residual = res;
     return 0;
@@ -1103,13 +1176,162 @@ int vk_gpe_set_advection(vk_gpe* h, int taylor_green, double vnx, double vny)
     if (!h || (taylor_green != 0 && taylor_green != 1) || !std::isfinite(vnx) || !std::isfinite(vny))
         return fail(VK_ERR_ARG, "set_advection: taylor_green must be 0 or 1 and the velocity finite");
     HIPCHK(hipStreamSynchronize(h->stream));
-    if ((taylor_green || vnx != 0.0 || vny != 0.0) && !h->Hdy) {
-        HIPCHK(hipMalloc(&h->Hdy, f2(h) * sizeof(cplx)));
-        HIPCHK(hipMemset(h->Hdy, 0, f2(h) * sizeof(cplx)));
+    if (taylor_green || vnx != 0.0 || vny != 0.0) {
+        int rc = ensure_hdy(h);
+        if (rc) return rc;
     }
     h->adv_tg = taylor_green;
     h->adv_vx = vnx;
     h->adv_vy = vny;
+    return 0;
+}
+
+static int ensure_hdy(vk_gpe* h)
+{
+    if (!h->Hdy) {
+        HIPCHK(hipMalloc(&h->Hdy, f2(h) * sizeof(cplx)));
+        HIPCHK(hipMemset(h->Hdy, 0, f2(h) * sizeof(cplx)));
+    }
+    return 0;
+}
+
+static int ensure_field_bufs(vk_gpe* h)
+{
+    if (!h->Ux) HIPCHK(hipMalloc(&h->Ux, f2(h) * sizeof(double)));
+    if (!h->Uy) HIPCHK(hipMalloc(&h->Uy, f2(h) * sizeof(double)));
+    return ensure_hdy(h);
+}
+
+int vk_gpe_set_advection_field(vk_gpe* h, const double* ux, const double* uy)
+{
+    if (!h || (!ux) != (!uy)) return fail(VK_ERR_ARG, "set_advection_field: both components or neither");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!ux) {
+        h->adv_field = 0;
+        return 0;
+    }
+    int rc = ensure_field_bufs(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->Ux, ux, f2(h) * sizeof(double), hipMemcpyDefault, h->stream));
+    HIPCHK(hipMemcpyAsync(h->Uy, uy, f2(h) * sizeof(double), hipMemcpyDefault, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->adv_field = 1;
+    return 0;
+}
+
+int vk_gpe_set_vortex_advection(vk_gpe* h, int nv, const double* xs, const double* ys, const int* qs, int nimg)
+{
+    if (!h || nv <= 0 || !xs || !ys || !qs || nimg < 0) return fail(VK_ERR_ARG, "set_vortex_advection: bad arguments");
+    for (int j = 0; j < nv; ++j)
+        if (qs[j] != 1 && qs[j] != -1) return fail(VK_ERR_ARG, "set_vortex_advection: charges must be +-1");
+    int rc = ensure_field_bufs(h);
+    if (rc) return rc;
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * 2 * nv + sizeof(int) * nv));
+    HIPCHK(hipMemcpy(d, xs, sizeof(double) * nv, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + nv, ys, sizeof(double) * nv, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + 2 * nv, qs, sizeof(int) * nv, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(vk::k_vortex_velocity, dim3(1024), dim3(256), 0, h->stream, h->Ux, h->Uy, h->N, nv,
+                       (const double*)d, (const double*)(d + nv), (const int*)(d + 2 * nv), h->p.xi, nimg,
+                       h->p.alpha);
+    const hipError_t e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    HIPCHK(e);
+    HIPCHK(hipGetLastError());
+    h->adv_field = 1;
+    return 0;
+}
+
+int vk_vortex_velocity_host(int N, int nv, const double* xs, const double* ys, const int* qs, double xi,
+                            double alpha, int nimg, double* ux, double* uy)
+{
+    if (N <= 0 || nv <= 0 || !xs || !ys || !qs || !ux || !uy || nimg < 0)
+        return fail(VK_ERR_ARG, "vortex_velocity_host: bad arguments");
+    const double dx = 2.0 * 3.14159265358979323846 / N;
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i)
+            vk::vortex_velocity_point(i * dx, j * dx, nv, xs, ys, qs, xi, nimg, alpha, ux + (size_t)j * N + i,
+                                      uy + (size_t)j * N + i);
+    return 0;
+}
+
+int vk_gpe_set_sgle(vk_gpe* h, int on, double oneybeta, double nuN, double chempot0, double ktrunc,
+                    unsigned long long seed)
+{
+    if (!h || (on != 0 && on != 1) || !(oneybeta >= 0.0) || !std::isfinite(nuN) || !std::isfinite(chempot0) ||
+        !std::isfinite(ktrunc) || !std::isfinite(oneybeta))
+        return fail(VK_ERR_ARG, "set_sgle: on must be 0 or 1, oneybeta >= 0 and the parameters finite");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!on) {
+        h->sgle = 0;
+        return 0;
+    }
+    int rc = need_x0(h);
+    if (rc) return rc;
+    if (!h->Xn) HIPCHK(hipMalloc(&h->Xn, f2(h) * sizeof(cplx)));
+    if (!h->Kn) HIPCHK(hipMalloc(&h->Kn, f2(h) * sizeof(cplx)));
+    if (!h->omega_dev) HIPCHK(hipMalloc(&h->omega_dev, sizeof(double)));
+    if (!h->npart) HIPCHK(hipMalloc(&h->npart, sizeof(double) * kNormBlocks));
+    HIPCHK(hipMemcpy(h->omega_dev, &chempot0, sizeof(double), hipMemcpyHostToDevice));
+    h->p.omega = chempot0;
+    h->sgle = 1;
+    h->sg_beta = oneybeta;
+    h->sg_nuN = nuN;
+    h->sg_ktrunc = ktrunc;
+    h->sg_seed = seed;
+    h->sg_draw = 0;
+    return 0;
+}
+
+// the chemical potential the SGLE steps have reached, into the handle's
+// parameters (RK4 after SGLE runs at it, as the reference's omega_chem)
+static int sgle_sync_omega(vk_gpe* h)
+{
+    if (!h->sgle) return 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(&h->p.omega, h->omega_dev, sizeof(double), hipMemcpyDeviceToHost));
+    if (h->e1R) {
+        const vk::EnsReal x = ens_entry(h);
+        HIPCHK(hipMemcpy(h->e1R, &x, sizeof(x), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int vk_gpe_get_chempot(vk_gpe* h, double* omega)
+{
+    if (!h || !omega) return fail(VK_ERR_ARG, "get_chempot: null argument");
+    int rc = sgle_sync_omega(h);
+    if (rc) return rc;
+    *omega = h->p.omega;
+    return 0;
+}
+
+int vk_gpe_argle_step_noise(vk_gpe* h, const double* noise_xy, double* residual)
+{
+    if (!h || !noise_xy) return fail(VK_ERR_ARG, "argle_step_noise: null argument");
+    if (!h->sgle) return fail(VK_ERR_STATE, "argle_step_noise: SGLE is off (vk_gpe_set_sgle)");
+    int rc = ensure_full(h);
+    if (rc) return rc;
+    if ((rc = argle_step(h, residual != nullptr, noise_xy))) return rc;
+    if ((rc = sgle_sync_omega(h))) return rc;
+    HIPCHK(hipGetLastError());
+    if (residual) {
+        double r[2];
+        HIPCHK(hipMemcpy(r, h->dsum, sizeof(r), hipMemcpyDeviceToHost));
+        *residual = std::sqrt(r[0]) / (h->p.dt * std::sqrt(r[1]));
+    }
+    return 0;
+}
+
+int vk_sgle_noise(vk_gpe* h, unsigned long long draw, double* out_xy)
+{
+    if (!h || !out_xy) return fail(VK_ERR_ARG, "sgle_noise: null argument");
+    if (!h->sgle) return fail(VK_ERR_STATE, "sgle_noise: SGLE is off (vk_gpe_set_sgle)");
+    hipLaunchKernelGGL(vk::k_sgle_noise, dim3(1024), dim3(256), 0, h->stream, h->Xn, h->N, draw, h->sg_seed, 1.0, 1.0,
+                       1);
+    HIPCHK(hipMemcpyAsync(out_xy, h->Xn, f2(h) * sizeof(cplx), hipMemcpyDefault, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -1373,7 +1595,8 @@ template <class C> void preconfig_passes()
     set_lds(vk::k_p1<C>, (size_t)C::TR * C::ROWRS * sizeof(double));
     set_lds(vk::k_p2<C>, (size_t)C::TC * C::COLRS * sizeof(double));
     set_lds(vk::k_p3<C, 0>, (size_t)C::TR * C::ROWRS * sizeof(double) + (size_t)C::S * C::ROWT * sizeof(double) + 16);
-    set_lds(vk::k_p4<C>, (size_t)C::TC * C::COLRS * sizeof(double));
+    for (auto k : {vk::k_p4<C, 1>, vk::k_p4<C, 2>, vk::k_p4<C, 3>, vk::k_p4<C, 4>})
+        set_lds(k, (size_t)C::TC * C::COLRS * sizeof(double));
 }
 
 vk::EnsReal ens_entry(const vk_gpe* h)

@@ -164,20 +164,25 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv(const cplx* __restri
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
+    const brsrc rH = mkbuf(H, N);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int kxi = t + T * s;
-        cplx a = czero();
+        unsigned o;
         if (pload == 2) {
-            if (kxi < Kp) a = H[hoff<C>(kxi / TC, y, kxi % TC)];
-            else if (kxi > N - Kp) {
-                const int km = N - kxi;
-                a = conjc(H[hoff<C>(km / TC, y, km % TC)]);
-            }
+            const int km = kxi >= Kp ? N - kxi : kxi;
+            o = boff(hoff<C>(km / TC, y, km % TC), kxi < Kp || kxi > N - Kp);
         } else {
             const int pp = VK_PANEL(s);
-            if (panel_stored(pp, NP, g, pload)) a = H[hoff<C>(pp, y, c)];
+            o = boff(hoff<C>(pp, y, c), panel_stored(pp, NP, g, pload));
         }
+        v[s] = bld(rH, o);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int kxi = t + T * s;
+        cplx a = v[s];
+        if (pload == 2 && kxi > N - Kp) a = conjc(a);
         if (op == 1) {
             const double kk = (double)ksg(kxi, N);
             a = mk(-kk * a.y, kk * a.x);
@@ -241,11 +246,14 @@ __device__ __forceinline__ void p1_body(int blk_, int nblk_, const cplx* __restr
 {
     VK_ROW_SETUP(C)
     cplx v[S];
+    const brsrc rH = mkbuf(Hpsi, N);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int pp = VK_PANEL(s);
-        v[s] = panel_stored(pp, NP, g, psi_pmode) ? conjc(Hpsi[hoff<C>(pp, y, c)]) : czero();
+        v[s] = bld(rH, boff(hoff<C>(pp, y, c), panel_stored(pp, NP, g, psi_pmode)));
     }
+#pragma unroll
+    for (int s = 0; s < S; ++s) v[s] = panel_stored(VK_PANEL(s), NP, g, psi_pmode) ? conjc(v[s]) : czero();
     p1_core<C, XPSI>(v, lg, t, y, pidx0, c, Hrho, tw, g, scale, Xpsi);
 }
 
@@ -505,6 +513,10 @@ __device__ __forceinline__ void pcoord(double a, double& X, double& dX)
 struct AdvArgs {
     int tg;
     double vx, vy, c4, sc;
+    // a caller-supplied velocity field in place of the Taylor-Green one
+    // (x-space, x fastest), when ux is set
+    const double* ux;
+    const double* uy;
 };
 
 // P3 (row): rho_d (+ i V) row from the kx >= 0 halves (Hermitian), psi row
@@ -537,19 +549,26 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     if constexpr ((flags & 2) != 0) {
         for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
     }
+    // kx >= 0 columns are stored; kx < 0 ones are the conjugates of N - kx
+    const brsrc rR = mkbuf(Hrhod, N);
+    cplx bv[(flags & 1) ? S : 1];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int kxi = t + T * s;
-        cplx a = czero(), b = czero();
-        if (kxi < Kp) {
-            const size_t o = hoff<C>(kxi / TC, y, kxi % TC);
-            a = Hrhod[o];
-            if constexpr (flags & 1) b = HV[o];
-        } else if (kxi > N - Kp) {
-            const int km = N - kxi;
-            const size_t o = hoff<C>(km / TC, y, km % TC);
-            a = conjc(Hrhod[o]);
-            if constexpr (flags & 1) b = conjc(HV[o]);
+        const bool neg = kxi >= Kp;
+        const int km = neg ? N - kxi : kxi;
+        const unsigned o = boff(hoff<C>(km / TC, y, km % TC), kxi < Kp || kxi > N - Kp);
+        v[s] = bld(rR, o);
+        if constexpr ((flags & 1) != 0) bv[(flags & 1) ? s : 0] = bld(mkbuf(HV, N), o);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const bool neg = t + T * s > N - Kp;
+        cplx a = v[s], b = czero();
+        if constexpr ((flags & 1) != 0) b = bv[(flags & 1) ? s : 0];
+        if (neg) {
+            a = conjc(a);
+            b = conjc(b);
         }
         v[s] = conjc(mk(a.x - b.y, a.y + b.x));   // a + i b, conj for the inverse
     }
@@ -659,7 +678,14 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     // Taylor-Green u at (x, y) from the twiddle table: tw[k] = e^{-2 pi i k/N}
     const double cy_ = tw[y].x, sy_ = -tw[y].y;
     if constexpr (ADV) {
-        if (aa.tg) {
+        if (aa.ux) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const size_t i = (size_t)y * N + t + T * s;
+                const double ux = aa.ux[i], uy = aa.uy[i];
+                VK_F(s) -= aa.c4 * (ux * ux + uy * uy);
+            }
+        } else if (aa.tg) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const cplx e = tw[t + T * s];
@@ -710,7 +736,10 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const cplx e = tw[t + T * s];
-                const double u = comp == 0 ? (aa.tg ? -e.y * cy_ : 0.0) + aa.vx : (aa.tg ? -e.x * sy_ : 0.0) + aa.vy;
+                double uf;
+                if (aa.ux) uf = (comp == 0 ? aa.ux : aa.uy)[(size_t)y * N + t + T * s];
+                else uf = aa.tg ? (comp == 0 ? -e.y * cy_ : -e.x * sy_) : 0.0;
+                const double u = uf + (comp == 0 ? aa.vx : aa.vy);
                 const cplx d = rmul(aa.sc, conjc(w[s]));   // d psi / dx_comp
                 v[s] = cadd(v[s], mk(u * d.y, -u * d.x));  // -i u d
             }
@@ -774,8 +803,10 @@ struct RKArgs {
 // (dt/3) rhs3, equals r (K2 - K0) with the K2 they write, and the next
 // stage rebuilds it from arrays it reads anyway (two fewer field writes
 // and one fewer field read per step; each costs one rounding of K2, ~1 ulp
-// of kpsi).
-template <class C>
+// of kpsi).  The stage is a compile-time constant, and the K accesses of
+// the dealiased modes go to the out-of-range offset (loads read zero,
+// stores are dropped), so the loads of all points are issued back to back.
+template <class C, int STAGE>
 __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
                                                 cplx* __restrict__ K2, const cplx* __restrict__ tw,
@@ -788,51 +819,57 @@ __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restr
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
     const int kxs = ksg(kx, N);
     const size_t rowo = (size_t)kx * N;
+    const brsrc r0 = mkbuf(K0, N), r1 = mkbuf(K1, N), r2 = mkbuf(K2, N);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int ky = t + T * s;
         const int kys = ksg(ky, N);
-        cplx kn = czero();
-        if (!kmasked(kxs, kys, N)) {
-            const size_t o = rowo + ky;
-            const double kr2 = (double)(kxs * kxs + kys * kys);
-            const cplx s1 = K0[o];
-            const cplx ps = a.stage == 1 ? s1 : K2[o];
-            const cplx lin = rmul(a.alpha * kr2 - a.omega, ps);
-            const cplx rhs = cmul(a.pref, cadd(lin, v[s]));
-            const cplx inc = rmul(a.wacc, rhs);
-            cplx acc = inc;
-            if (a.stage == 2) acc = cadd(rmul(a.r1, csub(ps, s1)), inc);
-            else if (a.stage == 3) acc = cadd(K1[o], inc);
-            else if (a.stage == 4) acc = cadd(cadd(K1[o], rmul(a.r3, csub(ps, s1))), inc);
-            if (a.stage < 4) {
-                if (a.stage == 2) K1[o] = acc;
-                kn = cadd(s1, rmul(a.cstep, rhs));
-                K2[o] = kn;
-            } else {
-                kn = cadd(s1, acc);
-                K0[o] = kn;
-            }
+        const bool keep = !kmasked(kxs, kys, N);
+        const unsigned o = boff(rowo + ky, keep);
+        const double kr2 = (double)(kxs * kxs + kys * kys);
+        const cplx s1 = bld(r0, o);
+        const cplx ps = STAGE == 1 ? s1 : bld(r2, o);
+        const cplx lin = rmul(a.alpha * kr2 - a.omega, ps);
+        const cplx rhs = cmul(a.pref, cadd(lin, v[s]));
+        const cplx inc = rmul(a.wacc, rhs);
+        cplx acc = inc, kn;
+        if constexpr (STAGE == 2) acc = cadd(rmul(a.r1, csub(ps, s1)), inc);
+        else if constexpr (STAGE == 3) acc = cadd(bld(r1, o), inc);
+        else if constexpr (STAGE == 4) acc = cadd(cadd(bld(r1, o), rmul(a.r3, csub(ps, s1))), inc);
+        if constexpr (STAGE < 4) {
+            if constexpr (STAGE == 2) bst(r1, o, acc);
+            kn = cadd(s1, rmul(a.cstep, rhs));
+            bst(r2, o, kn);
+        } else {
+            kn = cadd(s1, acc);
+            bst(r0, o, kn);
         }
-        v[s] = conjc(kn);
+        v[s] = conjc(keep ? kn : czero());
     }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
 }
 
-template <class C>
+template <class C, int STAGE>
 __global__ void __launch_bounds__(C::COLT) VK_OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
                                                 cplx* __restrict__ K2, const cplx* __restrict__ tw,
                                                 Geo g, RKArgs a)
 {
-    p4_body<C>(blockIdx.x, gridDim.x, HN, Hpsi, K0, K1, K2, tw, g, a);
+    p4_body<C, STAGE>(blockIdx.x, gridDim.x, HN, Hpsi, K0, K1, K2, tw, g, a);
 }
 
 struct ArgleArgs {
     double alpha, omega, dt;
     int want_res;
+    // SGLE (src/gpe.f90:200-217): with Kn set, sqrt(dt) Kn (dealiased) is
+    // added to the numerator, omega is read from *omega_dev (the chemical
+    // potential of this step) and, for ktrunc2 > 0, modes with
+    // k^2 > ktrunc2 are zeroed
+    const cplx* Kn;
+    const double* omega_dev;
+    double sdt, ktrunc2;
 };
 
 // ARGLE A4 (column, all panels): forward along y -> N-hat; implicit update
@@ -852,6 +889,7 @@ __device__ __forceinline__ void a4_body(int blk_, int nblk_, const cplx* __restr
     const int kxs = ksg(kx, N);
     const size_t rowo = (size_t)kx * N;
     double r[2] = {0.0, 0.0};
+    const double omega = a.omega_dev ? *a.omega_dev : a.omega;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int ky = t + T * s;
@@ -859,10 +897,15 @@ __device__ __forceinline__ void a4_body(int blk_, int nblk_, const cplx* __restr
         const size_t o = rowo + ky;
         const cplx old = K0[o];
         const double kr2 = (double)(kxs * kxs + kys * kys);
-        const cplx t1 = cadd(v[s], rmul(a.omega, old));
+        const cplx t1 = cadd(v[s], rmul(omega, old));
         const double t2 = 1.0 + a.alpha * kr2 * a.dt;
-        const cplx num = cadd(old, rmul(a.dt, t1));
-        const cplx nw = mk(num.x / t2, num.y / t2);
+        cplx num = cadd(old, rmul(a.dt, t1));
+        if (a.Kn) {
+            const cplx nt = kmasked(kxs, kys, N) ? czero() : rmul(a.sdt, a.Kn[o]);
+            num = cadd(num, nt);
+        }
+        cplx nw = mk(num.x / t2, num.y / t2);
+        if (a.ktrunc2 > 0.0 && kr2 > a.ktrunc2) nw = czero();
         K0[o] = nw;
         if (a.want_res) {
             const double dx = nw.x - old.x, dy = nw.y - old.y;
@@ -1257,7 +1300,97 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
 }
 
 // ----------------------------------------------------------------------------
-// Vortex initial condition
+// SGLE (src/gpe.f90:149-217, 226-266)
+// ----------------------------------------------------------------------------
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): ten rounds of two 32x32->64 multiplications
+// and a Weyl-sequence key schedule; counter c, key k.
+__host__ __device__ inline void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1)
+{
+    for (int r = 0; r < 10; ++r) {
+        const unsigned long long p0 = 0xD2511F53ull * c[0], p1 = 0xCD9E8D57ull * c[2];
+        const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+        const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+        const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Two independent standard normals at grid point i for noise draw `step`
+// (counter (i, 0, step_lo, step_hi), key = seed): uniforms from 53-bit
+// fractions of the 64-bit words, u1 in (0, 1], u2 in [0, 1), then the
+// Box-Muller transform.  The reference draws its pair with the polar method
+// from Fortran's RANDOM_NUMBER; only the distribution is the same.
+__host__ __device__ inline void sgle_normals(unsigned long long i, unsigned long long step, unsigned long long seed,
+                                             double* v1, double* v2)
+{
+    unsigned c[4] = {(unsigned)i, (unsigned)(i >> 32), (unsigned)step, (unsigned)(step >> 32)};
+    philox4x32_10(c, (unsigned)seed, (unsigned)(seed >> 32));
+    const unsigned long long a = ((unsigned long long)c[1] << 32) | c[0];
+    const unsigned long long b = ((unsigned long long)c[3] << 32) | c[2];
+    const double u1 = (double)((a >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(b >> 11) * 0x1.0p-53;
+    const double rr = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(6.283185307179586476925 * u2, &sn, &cs);
+    *v1 = rr * cs;
+    *v2 = rr * sn;
+}
+
+// X = noise field of draw `step`, times (1/sqrt(dvol)) then noisesigma
+// (src/gpe.f90:163), or the caller's standard normals in X scaled in place
+// (gen = 0)
+__global__ void k_sgle_noise(cplx* __restrict__ X, int N, unsigned long long step, unsigned long long seed,
+                             double invsq, double sigma, int gen)
+{
+    const size_t n2 = (size_t)N * N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        double v1, v2;
+        if (gen) sgle_normals(i, step, seed, &v1, &v2);
+        else {
+            v1 = X[i].x;
+            v2 = X[i].y;
+        }
+        X[i] = rmul(sigma, rmul(invsq, mk(v1, v2)));
+    }
+}
+
+// block partial sums of |K|^2 over a k-space field
+__global__ void k_knorm(const cplx* __restrict__ K, size_t n2, double* __restrict__ part)
+{
+    __shared__ double sh[16];
+    double acc[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        const cplx z = K[i];
+        acc[0] += z.x * z.x + z.y * z.y;
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// finiteT_adhoc_chempot (src/gpe.f90:255-264): norm = sum_x |psi|^2 dvol =
+// (dvol/N^2) sum_k |psi-hat|^2 (Parseval), omega -= dt (nuN/L^2)(norm - L^2)
+__global__ void k_chempot(const double* __restrict__ part, int nb, double* __restrict__ omega, double dvol_n2,
+                          double dt, double nuN, double L2)
+{
+    __shared__ double sh[16];
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc[0] += part[b];
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) {
+        const double norm = acc[0] * dvol_n2;
+        omega[0] = omega[0] - dt * (nuN / L2) * (norm - L2);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Vortex initial condition and the imposed vortex velocity
 // ----------------------------------------------------------------------------
 
 // Padé-core vortex product: amplitude prod_j sqrt(rho(r_j/xi)), r_j the
@@ -1297,6 +1430,52 @@ __host__ __device__ inline void ic_point(double x, double y, int nv, const doubl
     const double c = cos(th), s = sin(th);
     out[0] = amp * (pr * c - pim * s);
     out[1] = amp * (pr * s + pim * c);
+}
+
+// Imposed velocity of a vortex configuration for the advective ARGLE:
+// u = 2 alpha rho0 grad(theta0), the current of the Padé product ansatz
+// (rho0 = prod_j rho(r_j/xi), theta0 its phase), smooth and zero in the
+// cores.  grad arg(sin((X + i Y_m)/2)) = (-sinh Y_m, sin X)/(2 (cosh Y_m -
+// cos X)), summed over the nimg images Y_m = Y - 2 pi m and the vortices
+// with their charges, minus (sum_j q_j x_j)/(2 pi) along y from the linear
+// phase term.
+__host__ __device__ inline void vortex_velocity_point(double x, double y, int nv, const double* xs,
+                                                      const double* ys, const int* qs, double xi, int nimg,
+                                                      double alpha, double* ux, double* uy)
+{
+    const double pi = 3.14159265358979323846;
+    const double a1 = 11.0 / 32.0, a2 = 11.0 / 384.0, b1 = 1.0 / 3.0;
+    double rho = 1.0, gx = 0.0, gy = 0.0, lin = 0.0;
+    for (int j = 0; j < nv; ++j) {
+        const double dxp = x - xs[j], dyp = y - ys[j];
+        const double wx = dxp - 2.0 * pi * floor(dxp / (2.0 * pi) + 0.5);
+        const double wy = dyp - 2.0 * pi * floor(dyp / (2.0 * pi) + 0.5);
+        const double r2 = (wx * wx + wy * wy) / (xi * xi);
+        rho *= r2 * (a1 + a2 * r2) / (1.0 + b1 * r2 + a2 * r2 * r2);
+        lin += qs[j] * xs[j];
+        const double sX = sin(dxp), cX = cos(dxp);
+        for (int m = -nimg; m <= nimg; ++m) {
+            const double Ym = dyp - 2.0 * pi * m;
+            const double den = cosh(Ym) - cX;
+            if (den > 0.0) {
+                gx += qs[j] * (-0.5 * sinh(Ym) / den);
+                gy += qs[j] * (0.5 * sX / den);
+            }
+        }
+    }
+    gy -= lin / (2.0 * pi);
+    *ux = 2.0 * alpha * rho * gx;
+    *uy = 2.0 * alpha * rho * gy;
+}
+
+__global__ void k_vortex_velocity(double* __restrict__ UX, double* __restrict__ UY, int N, int nv,
+                                  const double* __restrict__ xs, const double* __restrict__ ys,
+                                  const int* __restrict__ qs, double xi, int nimg, double alpha)
+{
+    const size_t n2 = (size_t)N * N;
+    const double dx = 2.0 * 3.14159265358979323846 / N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        vortex_velocity_point((int)(i % N) * dx, (int)(i / N) * dx, nv, xs, ys, qs, xi, nimg, alpha, UX + i, UY + i);
 }
 
 __global__ void k_ic(cplx* __restrict__ X, int N, int nv, const double* __restrict__ xs,

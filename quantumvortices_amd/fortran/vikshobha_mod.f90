@@ -160,6 +160,25 @@ MODULE vikshobha_c
       INTEGER(C_INT), VALUE :: nrst, np
       REAL(C_DOUBLE) :: pos(*), vel(*), fext(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_set_sgle(h, on, oneybeta, nuN, chempot0, ktrunc, seed) BIND(C, NAME='vk_gpe_set_sgle')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, C_LONG_LONG
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: on
+      REAL(C_DOUBLE), VALUE :: oneybeta, nuN, chempot0, ktrunc
+      INTEGER(C_LONG_LONG), VALUE :: seed
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_get_chempot(h, omega) BIND(C, NAME='vk_gpe_get_chempot')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: omega
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_gpe_set_vortex_advection(h, nv, xs, ys, qs, nimg) BIND(C, NAME='vk_gpe_set_vortex_advection')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: nv, nimg
+      REAL(C_DOUBLE) :: xs(*), ys(*)
+      INTEGER(C_INT) :: qs(*)
+    END FUNCTION
     FUNCTION vk_last_error() BIND(C, NAME='vk_last_error')
       IMPORT :: C_PTR
       TYPE(C_PTR) :: vk_last_error
@@ -204,6 +223,13 @@ MODULE vikshobha_gpu
   REAL(C_DOUBLE) :: ppotV0 = 0.0_C_DOUBLE, ppotd = 0.0_C_DOUBLE, gp_mass_part = 1.0_C_DOUBLE
   REAL(C_DOUBLE), ALLOCATABLE :: gp_pos_obj(:,:), gp_vel_obj(:,:), gp_f0ext_obj(:,:), gp_frc_obj(:,:)
   REAL(C_DOUBLE) :: eppot = 0.0_C_DOUBLE, eobj_kin = 0.0_C_DOUBLE, esrep = 0.0_C_DOUBLE, mom_obj(2) = 0.0_C_DOUBLE
+
+  ! SGLE (GPE_SGLE, namelist gpfiniteT, src/driver.f90:59): oneybeta, nuN,
+  ! chempot0, and GPE_SELFTRUNC's ktrunc; omega_chem follows the chemical
+  ! potential the steps reach
+  REAL(C_DOUBLE) :: oneybeta = 0.0_C_DOUBLE, nuN = 0.0_C_DOUBLE, chempot0 = 0.0_C_DOUBLE
+  REAL(C_DOUBLE) :: ktrunc = 0.0_C_DOUBLE, omega_chem = 0.0_C_DOUBLE
+  INTEGER(C_LONG_LONG) :: sgle_seed = 0_C_LONG_LONG
 
   INTERFACE initialize_fft
     MODULE PROCEDURE initialize_fft_namelist, initialize_fft_args
@@ -311,6 +337,27 @@ CONTAINS
   SUBROUTINE ARGLE(time)
     REAL(C_DOUBLE), INTENT(IN) :: time
     CALL vk_check(vk_gpe_argle_steps(vkh, 1_C_INT, argle_residual), 'ARGLE')
+    CALL vk_check(vk_gpe_get_chempot(vkh, omega_chem), 'ARGLE')
+  END SUBROUTINE
+
+  ! the GPE_SGLE build of ARGLE on the gpfiniteT globals (noise from the
+  ! library's generator, seeded by sgle_seed)
+  SUBROUTINE set_sgle(on)
+    LOGICAL, INTENT(IN) :: on
+    INTEGER(C_INT) :: o
+    o = 0
+    IF (on) o = 1
+    CALL vk_check(vk_gpe_set_sgle(vkh, o, oneybeta, nuN, chempot0, ktrunc, sgle_seed), 'set_sgle')
+    omega_chem = chempot0
+  END SUBROUTINE
+
+  ! ARGLE advected by the imposed velocity of a vortex configuration (the
+  ! current of its Pade product ansatz), which keeps the vortices in place
+  SUBROUTINE set_vortex_advection(nv, xs, ys, qs)
+    INTEGER, INTENT(IN) :: nv
+    REAL(C_DOUBLE), INTENT(IN) :: xs(nv), ys(nv)
+    INTEGER(C_INT), INTENT(IN) :: qs(nv)
+    CALL vk_check(vk_gpe_set_vortex_advection(vkh, INT(nv, C_INT), xs, ys, qs, 3_C_INT), 'set_vortex_advection')
   END SUBROUTINE
 
   ! diagnostics_gpe (src/diagnostics.f90:4-571): energies, norm, momentum and

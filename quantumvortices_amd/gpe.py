@@ -165,6 +165,55 @@ class GPE:
         and/or a constant counterflow (GPE_COUNTERFLOW, vnx, vny)."""
         nat.check(nat.lib().vk_gpe_set_advection(self.h, 1 if taylor_green else 0, float(vnx), float(vny)))
 
+    def set_advection_field(self, ux=None, uy=None):
+        """ARGLE advection by a supplied velocity field (x-space (N, N) real
+        arrays, host or device) in place of the Taylor-Green one; None
+        turns it off."""
+        if ux is None:
+            nat.check(nat.lib().vk_gpe_set_advection_field(self.h, None, None))
+            return
+        if not hasattr(ux, "data_ptr"):
+            ux = np.ascontiguousarray(ux, dtype=np.float64)
+            uy = np.ascontiguousarray(uy, dtype=np.float64)
+        nat.check(nat.lib().vk_gpe_set_advection_field(self.h, nat.dptr(ux), nat.dptr(uy)))
+
+    def set_vortex_advection(self, xs, ys, qs, nimg=3):
+        """Imposed velocity of a vortex configuration (the current of its
+        Padé product ansatz) as ARGLE's advection field."""
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        ys = np.ascontiguousarray(ys, dtype=np.float64)
+        qs = np.ascontiguousarray(qs, dtype=np.int32)
+        nat.check(nat.lib().vk_gpe_set_vortex_advection(self.h, len(xs), nat.ptr(xs), nat.ptr(ys),
+                                                        qs.ctypes.data_as(nat.ip), int(nimg)))
+
+    def set_sgle(self, on=True, oneybeta=0.0, nuN=0.0, chempot0=None, ktrunc=0.0, seed=0):
+        """SGLE (GPE_SGLE build of ARGLE): noise strength oneybeta,
+        chemical-potential rate nuN from chempot0 (default: the handle's
+        omega), ktrunc > 0 truncation, generator seed."""
+        c0 = self.params.omega if chempot0 is None else chempot0
+        nat.check(nat.lib().vk_gpe_set_sgle(self.h, 1 if on else 0, float(oneybeta), float(nuN), float(c0),
+                                            float(ktrunc), int(seed)))
+
+    def sgle_step(self, noise):
+        """One SGLE step with the caller's complex standard normals
+        noise[y, x]; returns the residual."""
+        a = noise if hasattr(noise, "data_ptr") else np.ascontiguousarray(noise, dtype=np.complex128)
+        r = np.zeros(1)
+        nat.check(nat.lib().vk_gpe_argle_step_noise(self.h, nat.dptr(a), nat.ptr(r)))
+        return float(r[0])
+
+    def sgle_noise(self, draw):
+        """Draw `draw` of the SGLE generator (unscaled complex normals)."""
+        out = np.empty((self.N, self.N), dtype=np.complex128)
+        nat.check(nat.lib().vk_sgle_noise(self.h, int(draw), nat.ptr(out)))
+        return out
+
+    @property
+    def chempot(self):
+        o = np.zeros(1)
+        nat.check(nat.lib().vk_gpe_get_chempot(self.h, nat.ptr(o)))
+        return float(o[0])
+
     def set_dt(self, dt):
         """Replace dt_fix of the live state."""
         nat.check(nat.lib().vk_gpe_set_dt(self.h, float(dt)))
@@ -475,6 +524,19 @@ def ic_vortices_host(N, xs, ys, qs, xi, nimg=3):
     nat.check(nat.lib().vk_ic_vortices_host(int(N), len(xs), nat.ptr(xs), nat.ptr(ys),
                                             qs.ctypes.data_as(nat.ip), float(xi), int(nimg), nat.ptr(out)))
     return out
+
+
+def vortex_velocity_host(N, xs, ys, qs, xi, alpha, nimg=3):
+    """Imposed velocity u = 2 alpha rho0 grad(theta0) of a vortex list on
+    the CPU (the field vk_gpe_set_vortex_advection builds on the device)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    qs = np.ascontiguousarray(qs, dtype=np.int32)
+    ux = np.empty((N, N))
+    uy = np.empty((N, N))
+    nat.check(nat.lib().vk_vortex_velocity_host(int(N), len(xs), nat.ptr(xs), nat.ptr(ys), qs.ctypes.data_as(nat.ip),
+                                                float(xi), float(alpha), int(nimg), nat.ptr(ux), nat.ptr(uy)))
+    return ux, uy
 
 
 def lattice_vortices(n=4, L=2.0 * math.pi):

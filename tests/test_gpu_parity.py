@@ -425,7 +425,8 @@ def test_particles_argle_vs_oracle(G, oracle_mod, ppotd):
 
 def test_prefetching_p1_equals_plain_p1(monkeypatch):
     """At 4096^2 the RK4 P1 runs as a persistent kernel that prefetches the
-    next row pair into LDS; VK_NO_P1_PREFETCH=1 selects the plain row pass.
+    next row pair into LDS when VK_P1_PREFETCH=1 (the default is the plain
+    row pass).
     Both compute the same transforms, so 5 RK4 steps with particles agree to
     roundoff (the two kernels may contract multiply-adds differently)."""
     from quantumvortices_amd import gpe as G
@@ -435,8 +436,8 @@ def test_prefetching_p1_equals_plain_p1(monkeypatch):
     xs, ys, qs = G.random_vortices(64, seed=5)
     pos = np.random.default_rng(6).uniform(0, 2 * np.pi, (8, 2))
     out = []
-    for off in ("0", "1"):
-        monkeypatch.setenv("VK_NO_P1_PREFETCH", off)
+    for on in ("1", "0"):
+        monkeypatch.setenv("VK_P1_PREFETCH", on)
         g = G.GPE(G.params_from_namelist(N, 2.5e-4, 2.5, **kw))
         g.set_ic_vortices(xs, ys, qs, nimg=3)
         g.initialize_particles(pos)

@@ -215,3 +215,42 @@ def test_imposed_vortex_velocity_keeps_vortices(oracle_mod):
         vp, xp, yp = _core(np.abs(psi_p), xv, yv, dx)
         assert math.hypot(xp - xv, yp - yv) > 3 * dx
     assert r < 1e-3
+
+
+def test_philox_known_answers():
+    """The generator's reference restatement against the published
+    Philox4x32-10 known-answer vectors (Random123 kat_vectors)."""
+    from philox_ref import philox4x32_10
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, out in kat:
+        got = philox4x32_10(*[np.array([v], np.uint32) for v in ctr], *key)
+        assert tuple(int(g[0]) for g in got) == out
+
+
+def test_vortex_velocity_host_matches_phase_gradient():
+    """The library's imposed vortex velocity (analytic gradient of the
+    product phase, times the Padé density) against central differences of
+    the phase, away from the cores."""
+    from quantumvortices_amd import gpe as G
+    N = 64
+    rng = np.random.default_rng(3)
+    xs, ys = rng.uniform(0, 2 * np.pi, 6), rng.uniform(0, 2 * np.pi, 6)
+    qs = np.array([1, -1, 1, -1, 1, -1], dtype=np.int32)
+    xi, alpha = 2.5 * 2 * np.pi / N, 0.0376
+    ux, uy = G.vortex_velocity_host(N, xs, ys, qs, xi, alpha)
+    fx, fy = vortex_velocity(N, xs, ys, qs, alpha)
+    rho0 = np.abs(G.ic_vortices_host(N, xs, ys, qs, xi)) ** 2
+    x = 2 * np.pi * np.arange(N) / N
+    X, Y = np.meshgrid(x, x)
+    far = np.ones((N, N), bool)
+    for xv, yv in zip(xs, ys):
+        dx = (X - xv + np.pi) % (2 * np.pi) - np.pi
+        dy = (Y - yv + np.pi) % (2 * np.pi) - np.pi
+        far &= dx * dx + dy * dy > (3 * xi) ** 2
+    sc = np.abs(fx * rho0)[far].max()
+    assert np.abs(ux - fx * rho0)[far].max() < 1e-7 * sc
+    assert np.abs(uy - fy * rho0)[far].max() < 1e-7 * sc
+    assert np.all(np.isfinite(ux)) and np.all(np.isfinite(uy))
