@@ -180,12 +180,25 @@ def ensemble_bench(G, torch, steps, warmup):
     mode, graph = e.mode()
     points = sum(N * N for N in sizes)
     e.close()
+    # compulsory HBM bytes of one step of every member: the four passes of
+    # the four stages, per member size (the members run the same pass
+    # bodies as a single field of that size)
+    per_size = {}
+    for N in sorted(set(sizes)):
+        g1 = G.GPE(G.params_from_namelist(N, 1e-3, 2.5))
+        pb = g1.pass_bytes()
+        per_size[N] = 4.0 * sum(pb[k] for k in ("p1_rho_row", "p2_dealias_col", "p3_nonlin_row", "p4_rk4_col"))
+        g1.close()
+    step_bytes = sum(per_size[N] for N in sizes)
+    ach = step_bytes / (ms * 1e-3) / 1e9
     names = ["all sizes per launch", "one launch per size", "one launch per realization"]
     return {"workload": "384 realizations (128 x 128^2, 128 x 256^2, 128 x 512^2), 8-64 vortices, RK4 dt=1e-3",
             "ensemble_steps_per_s": round(1e3 / ms, 2), "ms_per_step": round(ms, 4),
             "realization_steps_per_s": round(len(sizes) * 1e3 / ms, 1),
             "grid_point_steps_per_s": float("%.4g" % (points * 1e3 / ms)),
             "schedule": {"launch": names[mode], "graph": graph},
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "compulsory_bytes_per_step": step_bytes},
             "autotune_ms_per_step": {names[m]: {"graph": round(float(tune[m][0]), 4), "direct": round(float(tune[m][1]), 4)}
                                      for m in range(3)}}
 
@@ -245,6 +258,30 @@ def argle_bench(G, torch, n_cfg, plain_cap):
         "surrogate_raw_rel_err": [float("%.3g" % r["raw_rel_err"]) for r in rows],
         "tf_raw_rel_err": [float("%.3g" % r["tf_raw_rel_err"]) for r in rows],
     }
+
+
+def diagnostics_bench(G, torch):
+    """diagnostics_gpe (energies, norm, momentum and the six shell spectra)
+    at 1024^2 (64 seeded vortices, the 1024^2 configuration's field) and on
+    the 4096^2 workload field: wall time of one synchronous call (kernels,
+    the host copies of the scalars and spectra, and the sync), after one
+    untimed call that allocates the diagnostics' work fields."""
+    out = {}
+    for N, nv, seed in ((1024, 64, 1), (4096, 256, 3)):
+        p = G.params_from_namelist(N, 5e-4, 2.5)
+        g = G.GPE(p)
+        xs, ys, qs = G.random_vortices(nv, seed)
+        g.set_ic_vortices(xs, ys, qs, nimg=3)
+        g.diagnostics_gpe()
+        torch.cuda.synchronize()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.diagnostics_gpe()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out["%d" % N] = {"ms_per_call": round(ms, 3), "shells": g.nshell}
+        g.close()
+    return out
 
 
 def sts_bench(G, torch):
@@ -332,6 +369,7 @@ def main():
     ap.add_argument("--argle-configs", type=int, default=2)
     ap.add_argument("--argle-plain-cap", type=int, default=400000)
     ap.add_argument("--no-sts", action="store_true")
+    ap.add_argument("--no-diag", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only (the single-field path would shard as "
@@ -400,6 +438,12 @@ def main():
         argle = argle_bench(G, torch, args.argle_configs, args.argle_plain_cap)
         log("bench: argle", argle)
 
+    diag = None
+    if not args.no_diag:
+        log("bench: diagnostics at 1024^2 and 4096^2")
+        diag = diagnostics_bench(G, torch)
+        log("bench: diagnostics", diag)
+
     sts = None
     if not args.no_sts:
         log("bench: spatio-temporal spectra 128^2 x 2048")
@@ -445,6 +489,7 @@ def main():
         "ensemble": ens,
         "argle_warm_start": argle,
         "spatio_temporal_spectra": sts,
+        "diagnostics": diag,
     }
     print(json.dumps(out), flush=True)
 

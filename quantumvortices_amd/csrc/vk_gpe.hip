@@ -346,6 +346,20 @@ int fwd2d(vk_gpe* h, const cplx* X, cplx* Hs, cplx* K, int op_row, int op_col)
     });
 }
 
+// forward 2D transform of a real field: only the kx >= 0 panels are stored
+// by the row pass and transformed by the column pass (the rest follows from
+// Hermitian symmetry)
+int fwd2d_half(vk_gpe* h, const cplx* X, cplx* Hs, cplx* K, int op_row, int op_col)
+{
+    return dispatch(h->N, [&](auto n) {
+        using C = CfgN<decltype(n)::value>;
+        row_launch<C>(h, PK_OTHER, vk::k_row_fwd<C>, X, Hs, (const cplx*)h->tw, h->geo, 2, op_row);
+        col_launch<C>(h, PK_OTHER, h->geo.Pp, vk::k_col_fwd<C>, (const cplx*)Hs, K, (const cplx*)h->tw, h->geo, 2,
+                      op_col);
+        return 0;
+    });
+}
+
 int col_inv(vk_gpe* h, const cplx* K, cplx* Hs, int pmode, int op)
 {
     return dispatch(h->N, [&](auto n) {
@@ -746,15 +760,18 @@ int diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal)
     if ((rc = row_inv(h, h->Hpsi, X[1], pl, 1, sc))) return rc;
     if ((rc = col_inv(h, h->K0, h->Hd, pl, 2))) return rc;
     if ((rc = row_inv(h, h->Hd, X[2], pl, 0, sc))) return rc;
-    // rho-hat (dealiased) -> Kd[4]; P_G rho -> X[3]
-    if ((rc = fwd2d(h, X[0], h->Hd, Kd[4], 1, 1))) return rc;
-    if ((rc = col_inv(h, Kd[4], h->Hd, 0, 0))) return rc;
-    if ((rc = row_inv(h, h->Hd, X[3], 0, 0, sc))) return rc;
+    // rho-hat (dealiased) -> Kd[4]; P_G rho -> X[3].  rho, P_G rho, V and
+    // the w and q fields are real: only their kx >= 0 columns are
+    // transformed, the rest is their Hermitian mirror.
+    const int Kp = h->TC * h->geo.Pp;
+    if ((rc = fwd2d_half(h, X[0], h->Hd, Kd[4], 1, 1))) return rc;
+    if ((rc = col_inv(h, Kd[4], h->Hd, 2, 0))) return rc;
+    if ((rc = row_inv(h, h->Hd, X[3], 2, 0, sc))) return rc;
     const int parts = h->have_part && h->p.np > 0;
     if (parts) {
-        particle_kv(h, h->N);   // all rows
-        if ((rc = col_inv(h, h->KV, h->Hd, 0, 0))) return rc;
-        if ((rc = row_inv(h, h->Hd, X[4], 0, 0, sc))) return rc;
+        particle_kv(h, Kp);
+        if ((rc = col_inv(h, h->KV, h->Hd, 2, 0))) return rc;
+        if ((rc = row_inv(h, h->Hd, X[4], 2, 0, sc))) return rc;
     }
     const double pi = 4.0 * std::atan(1.0);
     vk::DiagArgs da{h->p.alpha, h->p.omega, h->p.g, (double)1e-30f, h->N, parts};
@@ -762,17 +779,17 @@ int diagnostics(vk_gpe* h, double* scal, double* spec, double* pscal)
     hipLaunchKernelGGL(vk::k_diag_x, dim3(nb), dim3(256), 0, h->stream, (const cplx*)X[0], (const cplx*)X[1],
                        (const cplx*)X[2], (const cplx*)X[3], (const cplx*)X[4], X[1], X[2], X[5], X[6], da, h->dpart);
     hipLaunchKernelGGL(vk::k_reduce_fixed, dim3(1), dim3(256), 0, h->stream, (const double*)h->dpart, nb, 8, h->dsum);
-    // spectra: FFT of w_x, w_y, q_x, q_y
-    if ((rc = fwd2d(h, X[1], h->Hd, Kd[0], 0, 0))) return rc;
-    if ((rc = fwd2d(h, X[2], h->Hd, Kd[1], 0, 0))) return rc;
-    if ((rc = fwd2d(h, X[5], h->Hd, Kd[2], 0, 0))) return rc;
-    if ((rc = fwd2d(h, X[6], h->Hd, Kd[3], 0, 0))) return rc;
+    // spectra: FFT of w_x, w_y, q_x, q_y (kx >= 0 halves)
+    if ((rc = fwd2d_half(h, X[1], h->Hd, Kd[0], 0, 0))) return rc;
+    if ((rc = fwd2d_half(h, X[2], h->Hd, Kd[1], 0, 0))) return rc;
+    if ((rc = fwd2d_half(h, X[5], h->Hd, Kd[2], 0, 0))) return rc;
+    if ((rc = fwd2d_half(h, X[6], h->Hd, Kd[3], 0, 0))) return rc;
     const double en = 1.0 / ((double)h->N * (double)h->N);
     vk::SpecArgs sa{h->p.alpha, h->p.g, en * en, h->N, h->nshell};
     hipLaunchKernelGGL(vk::k_spectra, dim3(h->nshell + 1), dim3(256), 0, h->stream, (const int*)h->modes,
                        (const int*)h->offs, (const cplx*)Kd[0], (const cplx*)Kd[1], (const cplx*)Kd[2],
                        (const cplx*)Kd[3], (const cplx*)Kd[4], (const cplx*)h->K0, sa, h->dspec);
-    if (parts) particle_forces(h, Kd[4], h->N, 0, 0);
+    if (parts) particle_forces(h, Kd[4], Kp, 1, 0);
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipGetLastError());
     double s[8];

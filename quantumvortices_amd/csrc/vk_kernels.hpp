@@ -1016,8 +1016,14 @@ __global__ void k_spectra(const int* __restrict__ modes, const int* __restrict__
     for (int j = offs[m] + threadIdx.x; j < offs[m + 1]; j += blockDim.x) {
         const int idx = modes[j];
         const int kx = ksg(idx / a.N, a.N), ky = ksg(idx % a.N, a.N);
-        const double kxd = kx, kyd = ky, kr2 = kxd * kxd + kyd * kyd;
-        const cplx aa1 = KW1[idx], aa2 = KW2[idx];
+        // the real fields' transforms hold the kx >= 0 columns: a mode with
+        // kx < 0 takes the values of its mirror -k, whose terms are the same
+        // (every term is a squared modulus, invariant under k -> -k with
+        // the conjugated amplitudes)
+        const int kxm = kx < 0 ? -kx : kx, kym = kx < 0 ? -ky : ky;
+        const int idh = kxm * a.N + (kym < 0 ? kym + a.N : kym);
+        const double kxd = kxm, kyd = kym, kr2 = kxd * kxd + kyd * kyd;
+        const cplx aa1 = KW1[idh], aa2 = KW2[idh];
         cplx bb = czero();
         if (!(kx == 0 && ky == 0)) {
             // bb = -i (kx/k^2) aa1 - i (ky/k^2) aa2
@@ -1027,7 +1033,7 @@ __global__ void k_spectra(const int* __restrict__ modes, const int* __restrict__
         }
         const cplx cc1 = mk(-kxd * bb.y, kxd * bb.x), cc2 = mk(-kyd * bb.y, kyd * bb.x);
         const cplx dd1 = csub(aa1, cc1), dd2 = csub(aa2, cc2);
-        const cplx e1 = KQ1[idx], e2 = KQ2[idx], ff = KR[idx], gg = KP[idx];
+        const cplx e1 = KQ1[idh], e2 = KQ2[idh], ff = KR[idh], gg = KP[idx];
         auto ab2 = [](cplx z) { const double h = hypot(z.x, z.y); return h * h; };
         acc[0] += a.enorm * (ab2(aa1) + ab2(aa2)) * 0.5;
         acc[1] += a.enorm * (ab2(dd1) + ab2(dd2)) * 0.5;
