@@ -155,8 +155,8 @@ def cpu_baseline(G, N, dt, info, threads, psi_gpu):
 def ensemble_bench(G, torch, steps, warmup):
     """Batched ensemble: 384 independent realizations, 128 each at 128^2,
     256^2 and 512^2, with 8-64 zero-net-charge random vortices (seed =
-    realization id); RK4 dt=1e-3; schedule picked by the autotuner, one
-    step replayed as a HIP graph."""
+    realization id); RK4 dt=1e-3; launch schedule and member shapes picked
+    by the autotuners."""
     import numpy as np
     sizes = [128] * 128 + [256] * 128 + [512] * 128
     ps = [G.params_from_namelist(N, 1e-3, 2.5) for N in sizes]
@@ -167,6 +167,7 @@ def ensemble_bench(G, torch, steps, warmup):
         xs, ys, qs = G.random_vortices(nv, seed=r)
         e.set_ic_vortices(r, xs, ys, qs)
     tune = e.autotune(3)
+    shapes = e.autotune_shapes(3) if e.mode()[0] != 2 else {}
     ext = torch.cuda.ExternalStream(e.stream)
     e.rk4_async(warmup)
     torch.cuda.synchronize()
@@ -197,6 +198,9 @@ def ensemble_bench(G, torch, steps, warmup):
             "realization_steps_per_s": round(len(sizes) * 1e3 / ms, 1),
             "grid_point_steps_per_s": float("%.4g" % (points * 1e3 / ms)),
             "schedule": {"launch": names[mode], "graph": graph},
+            "shapes": {str(N): {"chosen": ("8 points per thread" if v[0] else "16 points per thread"),
+                                "ms_16pt": round(v[1], 4) if v[1] > 0 else None,
+                                "ms_8pt": round(v[2], 4) if v[2] > 0 else None} for N, v in shapes.items()},
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 4), "compulsory_bytes_per_step": step_bytes},
             "autotune_ms_per_step": {names[m]: {"graph": round(float(tune[m][0]), 4), "direct": round(float(tune[m][1]), 4)}
@@ -414,6 +418,21 @@ def main():
             "bytes_per_launch": pb[dom], "avg_ms": pt[dom]}
     per_pass = {k: {"ms": round(pt[k], 4), "GB/s": round(pb[k] / (pt[k] * 1e-3) / 1e9, 1) if pt[k] > 0 else None}
                 for k in passes}
+    # attainable bandwidth of the passes' access patterns on this GPU (no
+    # arithmetic): the ceilings below the 8 TB/s peak
+    import numpy as np
+    from quantumvortices_amd import _native as nat
+    pat = np.zeros(4)
+    nat.check(nat.lib().vk_pattern_bandwidth(20, nat.ptr(pat), 4))
+    attain = {"contiguous_copy": round(pat[0], 1), "row_pass_panel_gather": round(pat[1], 1),
+              "column_pass_panel_copy": round(pat[2], 1), "column_pass_one_column": round(pat[3], 1)}
+    ceiling = {"p1_rho_row": pat[1], "p2_dealias_col": pat[3], "p3_nonlin_row": pat[1], "p4_rk4_col": pat[2]}
+    for k in passes:
+        if per_pass[k]["GB/s"]:
+            per_pass[k]["frac_of_pattern_ceiling"] = round(per_pass[k]["GB/s"] / ceiling[k], 3)
+    roof["attainable_GBps"] = attain
+    roof["frac_of_pattern_ceiling"] = round(achieved / ceiling[dom], 4)
+    log("bench: attainable pattern bandwidth", attain)
     # step-level roofline: the compulsory bytes of the four passes of the
     # four stages (the particle kernels move a few MB), and the bytes the
     # counters measured for the same passes

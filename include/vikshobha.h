@@ -197,6 +197,12 @@ const char* vk_pass_name(int kind);
 int vk_pass_kinds(void);
 /* Compulsory HBM bytes one launch of kernel kind moves at this handle's N. */
 double vk_gpe_pass_bytes(vk_gpe* h, int kind);
+/* Measured attainable bandwidth (GB/s, read + written bytes per launch over
+ * the average of reps launches) of the passes' access patterns on a 4096^2
+ * complex128 field with no arithmetic: gbps[0] contiguous copy, [1] the
+ * row passes' gather of 64-byte panel pieces, [2] the column pass copying
+ * whole 2-column panels, [3] one column per workgroup (n <= 4 entries). */
+int vk_pattern_bandwidth(int reps, double* gbps, int n);
 
 /* Ensemble of independent realizations (no reference counterpart: the
  * reference runs one field per MPI job).  Members have N in {128, 256, 512,
@@ -220,6 +226,21 @@ int vk_ens_rk4_steps_async(vk_ens* e, int nsteps);
 int vk_ens_set_mode(vk_ens* e, int mode, int use_graph);
 int vk_ens_get_mode(vk_ens* e, int* mode, int* use_graph);
 int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out);
+/* Member shapes.  Sizes 256..1024 have two shapes with 256-thread
+ * workgroups: 0 (16 points per thread, Cfg<N>) and 1 (8 points per thread,
+ * radix-8 stages, half the panel width).  vk_ens_autotune_shapes times both
+ * for each such size in the current launch mode (one size at a time,
+ * ascending, kpsi restored before every trial), keeps the faster and
+ * remembers it per (N, GPU model) for ensembles created later in the
+ * process; vars_out[i] and ms_out[2i..2i+1] (ms/step of shapes 0, 1) are
+ * per distinct member size, ascending (-1: one shape only).  Members of
+ * shape 1 equal single-field runs to roundoff (the transforms factor
+ * differently), members of shape 0 bit for bit. */
+int vk_ens_set_shape(vk_ens* e, int N, int var);
+int vk_ens_get_shape(vk_ens* e, int N, int* var);
+int vk_ens_autotune_shapes(vk_ens* e, int nsteps, int* vars_out, double* ms_out);
+/* drop the remembered shapes: later ensembles start in shape 0 */
+void vk_ens_forget_shapes(void);
 
 /* Fourier neural operator for the ARGLE warm start (no reference counterpart:
  * the reference starts ARGLE from the analytic ansatz of

@@ -73,6 +73,7 @@ SIGNATURES = {
     "vk_pass_name": (ctypes.c_char_p, [ctypes.c_int]),
     "vk_pass_kinds": (ctypes.c_int, []),
     "vk_gpe_pass_bytes": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_pattern_bandwidth": (ctypes.c_int, [ctypes.c_int, dp, ctypes.c_int]),
     "vk_ens_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Params)]),
     "vk_ens_destroy": (None, [ctypes.c_void_p]),
     "vk_ens_size": (ctypes.c_int, [ctypes.c_void_p]),
@@ -86,6 +87,10 @@ SIGNATURES = {
     "vk_ens_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "vk_ens_get_mode": (ctypes.c_int, [ctypes.c_void_p, ip, ip]),
     "vk_ens_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp]),
+    "vk_ens_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "vk_ens_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ip]),
+    "vk_ens_autotune_shapes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ip, dp]),
+    "vk_ens_forget_shapes": (None, []),
     "vk_fno_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)] + [ctypes.c_int] * 7),
     "vk_fno_destroy": (None, [ctypes.c_void_p]),
     "vk_fno_nparams": (ctypes.c_long, [ctypes.c_void_p]),

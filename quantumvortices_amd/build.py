@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libvikshobha.so")
-SRC = [os.path.join(HERE, "csrc", f) for f in ("vk_gpe.hip", "vk_fno.hip", "vk_io.hip", "vk_sts.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("vk_gpe.hip", "vk_fno.hip", "vk_io.hip", "vk_sts.hip", "vk_patterns.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

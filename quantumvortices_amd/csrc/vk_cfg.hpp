@@ -61,6 +61,17 @@ template <> struct Cfg<2048> : CfgT<2048, 16, 4, 4, true> {};
 #endif
 template <> struct Cfg<4096> : CfgT<4096, VK4096_S, VK4096_TC, VK4096_TR, VK4096_SPLIT> {};
 
+// Shapes of the ensemble sizes, all with 256-thread workgroups
+// (TC*T = TR*T = 256): variant 0 is Cfg<N>; variant 1 holds 8 points per
+// thread instead of 16 (radix-8 stages, twice the threads per transform,
+// half the panel width).  N = 128 has only variant 0 (16 points per thread
+// would need 512-wide panels of 8 threads).
+template <int N, int V> struct CfgE : Cfg<N> {};
+template <> struct CfgE<256, 1> : CfgT<256, 8, 8, 8, true> {};
+template <> struct CfgE<512, 1> : CfgT<512, 8, 4, 4, true> {};
+template <> struct CfgE<1024, 1> : CfgT<1024, 8, 2, 2, true> {};
+constexpr bool has_variant(int N, int V) { return V == 0 || (V == 1 && (N == 256 || N == 512 || N == 1024)); }
+
 // Runtime panel geometry.  Retained |kx| <= M = floor(N/3) (the 2/3 rule
 // for 3 not dividing N).  Positive panels p < Pp cover kx in [0, M];
 // negative panels p >= NP - Pn cover kx in [N-M, N-1].
@@ -68,6 +79,13 @@ struct Geo {
     int M;
     int Pp, Pn;
 };
+
+// Row blocks are handed to the 8 XCDs round-robin; remap block b of nb so
+// that each XCD takes a contiguous run of row blocks (nb a multiple of 8).
+__device__ __forceinline__ int xcd_remap_blk(int b, int nb)
+{
+    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
 
 // Signed wavenumber of index i: (i-1) - N*(i/(N/2+1)) with 1-based i
 // (src/pspectral.f90:22); Nyquist maps to -N/2.

@@ -9,6 +9,7 @@ namespace vk {
 
 struct EnsReal {
     int N;
+    int var;        // shape variant (CfgE<N, var>)
     Geo g;
     cplx *K0, *K1, *K2, *Hpsi, *Hrho, *HN;
     const cplx* tw;
@@ -17,11 +18,14 @@ struct EnsReal {
 };
 
 #define VK_ENS_SWITCH(N, BODY)                                           \
-    switch (N) {                                                          \
-    case 128: { using C = Cfg<128>; BODY; break; }                        \
-    case 256: { using C = Cfg<256>; BODY; break; }                        \
-    case 512: { using C = Cfg<512>; BODY; break; }                        \
-    case 1024: { using C = Cfg<1024>; BODY; break; }                      \
+    switch (N * 2 + e.var) {                                              \
+    case 256: { using C = CfgE<128, 0>; BODY; break; }                    \
+    case 512: { using C = CfgE<256, 0>; BODY; break; }                    \
+    case 513: { using C = CfgE<256, 1>; BODY; break; }                    \
+    case 1024: { using C = CfgE<512, 0>; BODY; break; }                   \
+    case 1025: { using C = CfgE<512, 1>; BODY; break; }                   \
+    case 2048: { using C = CfgE<1024, 0>; BODY; break; }                  \
+    case 2049: { using C = CfgE<1024, 1>; BODY; break; }                  \
     default: break;                                                       \
     }
 

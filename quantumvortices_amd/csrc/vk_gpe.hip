@@ -152,6 +152,11 @@ struct vk_gpe {
     int *modes = nullptr, *offs = nullptr;
     double *dspec = nullptr, *dpart = nullptr;
     int diag_ready = 0;
+    // ensemble shape variant of this member (CfgE<N, ens_var>) and its
+    // panel geometry; Hpsi holds that variant's layout only between the
+    // ensemble's preparation and the member's next single-field access
+    int ens_var = 0;
+    Geo geo_v{};
     // one-member ensemble table: N <= 1024 runs the same kernels as a
     // batched ensemble member, so both give bit-identical results
     vk::EnsReal* e1R = nullptr;
@@ -1567,9 +1572,38 @@ struct vk_ens {
     int use_graph = 1;
     hipGraphExec_t gexec[3] = {};
     int synced = 0;
+    std::map<int, int> var;                       // N -> shape variant
 };
 
 namespace {
+
+// panel geometry of panel width TC (as vk_gpe_create computes it)
+Geo geo_of(int N, int TC)
+{
+    Geo g{};
+    const int NP = N / TC, M = N / 3;
+    g.M = M;
+    g.Pp = (M + 1 + TC - 1) / TC;
+    g.Pn = (M + TC - 1) / TC;
+    if (g.Pp + g.Pn > NP) { g.Pp = NP; g.Pn = 0; }
+    return g;
+}
+
+// call f with the shape type of (N, var)
+template <class F>
+int dispatch_var(int N, int var, F&& f)
+{
+    switch (N * 2 + var) {
+    case 256: return f(vk::CfgE<128, 0>{});
+    case 512: return f(vk::CfgE<256, 0>{});
+    case 513: return f(vk::CfgE<256, 1>{});
+    case 1024: return f(vk::CfgE<512, 0>{});
+    case 1025: return f(vk::CfgE<512, 1>{});
+    case 2048: return f(vk::CfgE<1024, 0>{});
+    case 2049: return f(vk::CfgE<1024, 1>{});
+    default: return fail(VK_ERR_UNSUPPORTED, "no such ensemble shape");
+    }
+}
 
 template <class C> void ens_lds(size_t* l)
 {
@@ -1590,8 +1624,9 @@ int build_maps(const std::vector<vk_gpe*>& hs, const std::vector<int>& ids, int*
         for (int r : ids) {
             const vk_gpe* h = hs[r];
             int TR = 0;
-            dispatch(h->N, [&](auto n) { TR = CfgN<decltype(n)::value>::TR; return 0; });
-            const int nb = k == 0 ? h->N / TR : (k == 1 ? h->geo.Pp : h->geo.Pp + h->geo.Pn);
+            dispatch_var(h->N, h->ens_var, [&](auto c) { TR = decltype(c)::TR; return 0; });
+            const Geo& gv = h->ens_var ? h->geo_v : h->geo;
+            const int nb = k == 0 ? h->N / TR : (k == 1 ? gv.Pp : gv.Pp + gv.Pn);
             se[2 * r] = (int)map.size();
             for (int b = 0; b < nb; ++b) map.push_back(r);
             se[2 * r + 1] = (int)map.size();
@@ -1620,7 +1655,8 @@ vk::EnsReal ens_entry(const vk_gpe* h)
 {
     vk::EnsReal x{};
     x.N = h->N;
-    x.g = h->geo;
+    x.var = h->ens_var;
+    x.g = h->ens_var ? h->geo_v : h->geo;
     x.K0 = h->K0; x.K1 = h->K1; x.K2 = h->K2; x.Hpsi = h->Hpsi; x.Hrho = h->Hrho; x.HN = h->HN;
     x.tw = h->tw;
     x.alpha = h->p.alpha;
@@ -1675,6 +1711,92 @@ int ens1_step(vk_gpe* h)
 
 }  // namespace
 
+namespace {
+
+// shape variant per (N, GPU model) chosen by vk_ens_autotune_shapes in this
+// process
+std::mutex g_shape_mu;
+std::map<std::pair<int, std::string>, int> g_shape_cache;
+
+std::string gpu_model()
+{
+    int dev = 0;
+    hipDeviceProp_t pr{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&pr, dev) != hipSuccess) return "?";
+    return std::string(pr.gcnArchName);
+}
+
+int cached_shape(int N)
+{
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shape_cache.find({N, gpu_model()});
+    return it == g_shape_cache.end() ? 0 : it->second;
+}
+
+void set_member_variant(vk_ens* e, int N, int v)
+{
+    e->var[N] = v;
+    for (int r : e->size_reals[N]) {
+        vk_gpe* h = e->h[r];
+        h->ens_var = v;
+        int TC = 0;
+        dispatch_var(N, v, [&](auto c) { TC = decltype(c)::TC; return 0; });
+        h->geo_v = geo_of(N, TC);
+    }
+}
+
+// (re)build the member table, the block maps of all launch modes and drop
+// captured graphs (after a shape change)
+int ens_rebuild(vk_ens* e)
+{
+    for (auto& g : e->gexec)
+        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+    for (int k = 0; k < 3; ++k) {
+        if (e->dmap[k]) { hipFree(e->dmap[k]); e->dmap[k] = nullptr; }
+        if (e->doff[k]) { hipFree(e->doff[k]); e->doff[k] = nullptr; }
+    }
+    for (auto& kv : e->smaps) for (auto p : kv.second) if (p) hipFree(p);
+    e->smaps.clear();
+    e->snblk.clear();
+    const int nreal = (int)e->h.size();
+    std::vector<vk::EnsReal> R(nreal);
+    for (int r = 0; r < nreal; ++r) R[r] = ens_entry(e->h[r]);
+    if (!e->dR) HIPCHK(hipMalloc(&e->dR, sizeof(vk::EnsReal) * nreal));
+    HIPCHK(hipMemcpy(e->dR, R.data(), sizeof(vk::EnsReal) * nreal, hipMemcpyHostToDevice));
+    std::vector<int> all(nreal);
+    for (int r = 0; r < nreal; ++r) all[r] = r;
+    int rc = build_maps(e->h, all, e->dmap, e->doff, e->nblk);
+    if (rc) return rc;
+    for (int N : e->sizes) {
+        std::array<int*, 6> m{};
+        int* dm[3];
+        int* dof[3];
+        std::array<int, 3> nb{};
+        if ((rc = build_maps(e->h, e->size_reals[N], dm, dof, nb.data()))) return rc;
+        for (int k = 0; k < 3; ++k) { m[2 * k] = dm[k]; m[2 * k + 1] = dof[k]; }
+        e->smaps[N] = m;
+        e->snblk[N] = nb;
+    }
+    e->synced = 0;
+    return 0;
+}
+
+// Hpsi = IFFT_y(K0) on the retained panels of shape (N, var)
+int col_inv_var(vk_gpe* h, int var)
+{
+    return dispatch_var(h->N, var, [&](auto c) {
+        using C = decltype(c);
+        const Geo& g = var ? h->geo_v : h->geo;
+        const size_t lds = (size_t)C::TC * C::COLRS * sizeof(double);
+        set_lds(vk::k_col_inv<C>, lds);
+        hipLaunchKernelGGL(vk::k_col_inv<C>, dim3(g.Pp + g.Pn), dim3(C::COLT), lds, h->stream, (const cplx*)h->K0,
+                           h->Hpsi, (const cplx*)h->tw, g, 1, 0);
+        return 0;
+    });
+}
+
+}  // namespace
+
 extern "C" {
 
 int vk_ens_create(vk_ens** out, int nreal, const vk_gpe_params* params)
@@ -1699,34 +1821,22 @@ int vk_ens_create(vk_ens** out, int nreal, const vk_gpe_params* params)
     for (auto& kv : e->size_reals) e->sizes.push_back(kv.first);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(VK_ERR_HIP, "ens stream"));
-    std::vector<vk::EnsReal> R(nreal);
-    for (int r = 0; r < nreal; ++r) {
-        vk_gpe* h = e->h[r];
-        R[r] = ens_entry(h);
-        dispatch(h->N, [&](auto n) { ens_lds<CfgN<decltype(n)::value>>(e->lds); return 0; });
-    }
-    if (hipMalloc(&e->dR, sizeof(vk::EnsReal) * nreal) != hipSuccess ||
-        hipMemcpy(e->dR, R.data(), sizeof(vk::EnsReal) * nreal, hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fail(VK_ERR_HIP, "ens table"));
-    std::vector<int> all(nreal);
-    for (int r = 0; r < nreal; ++r) all[r] = r;
-    int rc = build_maps(e->h, all, e->dmap, e->doff, e->nblk);
-    if (rc) return cleanup(rc);
-    for (int N : e->sizes) {
-        std::array<int*, 6> m{};
-        int* dm[3];
-        int* dof[3];
-        std::array<int, 3> nb{};
-        if ((rc = build_maps(e->h, e->size_reals[N], dm, dof, nb.data()))) return cleanup(rc);
-        for (int k = 0; k < 3; ++k) { m[2 * k] = dm[k]; m[2 * k + 1] = dof[k]; }
-        e->smaps[N] = m;
-        e->snblk[N] = nb;
-    }
-    for (int N : e->sizes) dispatch(N, [&](auto n) { preconfig_passes<CfgN<decltype(n)::value>>(); return 0; });
+    // LDS limits for every shape a member may take
+    for (int N : e->sizes)
+        for (int v = 0; v <= 1; ++v)
+            if (vk::has_variant(N, v)) dispatch_var(N, v, [&](auto c) { ens_lds<decltype(c)>(e->lds); return 0; });
     set_lds(vk::k_ens_p1, e->lds[0]);
     set_lds(vk::k_ens_p2, e->lds[1]);
     set_lds(vk::k_ens_p3, e->lds[2]);
     set_lds(vk::k_ens_p4, e->lds[3]);
+    for (int N : e->sizes) dispatch(N, [&](auto n) { preconfig_passes<CfgN<decltype(n)::value>>(); return 0; });
+    // shapes an earlier autotuning picked for this size on this GPU model
+    for (int N : e->sizes) {
+        const int v = cached_shape(N);
+        if (v > 0) set_member_variant(e, N, v);
+    }
+    int rc = ens_rebuild(e);
+    if (rc) return cleanup(rc);
     *out = e;
     return 0;
 }
@@ -1796,12 +1906,20 @@ int ens_step_launch(vk_ens* e)
     return 0;
 }
 
+// Every member's Hpsi = IFFT_y(K0) in its shape's layout (the one-launch
+// modes), or in the single-field layout (one launch per member).  A member
+// of shape variant 1 then marks its Hpsi stale for the single-field entry
+// points (hstate 0), which rebuild it in their layout on access.
 int ens_prepare(vk_ens* e)
 {
     if (e->synced) return 0;
     for (auto h : e->h) {
         int rc = ensure_rk4(h);
         if (rc) return rc;
+        if (e->mode != 2 && h->ens_var) {
+            if ((rc = col_inv_var(h, h->ens_var))) return rc;
+            h->hstate = 0;
+        }
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     e->synced = 1;
@@ -1871,6 +1989,7 @@ int vk_ens_get_psi(vk_ens* e, int r, double* psi, double* kpsi)
     vk_gpe* h;
     int rc = ens_member(e, r, &h);
     if (rc) return rc;
+    if (h->ens_var && psi) e->synced = 0;   // psi rebuilds Hpsi in the single-field layout
     return vk_gpe_get_psi(h, psi, kpsi);
 }
 
@@ -1879,7 +1998,30 @@ int vk_ens_diagnostics(vk_ens* e, int r, double* scal, double* spec, double* psc
     vk_gpe* h;
     int rc = ens_member(e, r, &h);
     if (rc) return rc;
+    if (h->ens_var) e->synced = 0;
     return vk_gpe_diagnostics(h, scal, spec, pscal);
+}
+
+void vk_ens_forget_shapes(void)
+{
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shape_cache.clear();
+}
+
+int vk_ens_set_shape(vk_ens* e, int N, int var)
+{
+    if (!e || !e->size_reals.count(N) || !vk::has_variant(N, var))
+        return fail(VK_ERR_ARG, "ens_set_shape: no members of that size, or no such shape for it");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    set_member_variant(e, N, var);
+    return ens_rebuild(e);
+}
+
+int vk_ens_get_shape(vk_ens* e, int N, int* var)
+{
+    if (!e || !var || !e->size_reals.count(N)) return fail(VK_ERR_ARG, "ens_get_shape: bad arguments");
+    *var = e->var.count(N) ? e->var[N] : 0;
+    return 0;
 }
 
 int vk_ens_rk4_steps_async(vk_ens* e, int nsteps)
@@ -1900,6 +2042,8 @@ int vk_ens_rk4_steps(vk_ens* e, int nsteps)
 int vk_ens_set_mode(vk_ens* e, int mode, int use_graph)
 {
     if (!e || mode < 0 || mode > 2) return fail(VK_ERR_ARG, "ens_set_mode: bad arguments");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (mode != e->mode) e->synced = 0;   // the per-member mode uses the single-field layout
     e->mode = mode;
     e->use_graph = use_graph ? 1 : 0;
     return 0;
@@ -1937,6 +2081,7 @@ int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
     int bm = 0, bg = 1;
     for (int m = 0; m < 3; ++m)
         for (int gph = 1; gph >= 0; --gph) {
+            if (m != e->mode) e->synced = 0;
             e->mode = m;
             e->use_graph = gph;
             if ((rc = ens_steps_async(e, 1))) return rc;   // warm (and capture)
@@ -1950,6 +2095,7 @@ int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
             if (ms_out) ms_out[2 * m + (1 - gph)] = per;
             if (per < best) { best = per; bm = m; bg = gph; }
         }
+    if (bm != e->mode) e->synced = 0;
     e->mode = bm;
     e->use_graph = bg;
     for (size_t i = 0; i < e->h.size(); ++i) {
@@ -1962,6 +2108,75 @@ int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
     hipEventDestroy(a);
     hipEventDestroy(b);
     return 0;
+}
+
+// Shape autotuner: for each member size with a second shape (8 points per
+// thread instead of 16), times whole-ensemble steps in the current launch
+// mode with either shape for that size (the other sizes as chosen so far,
+// sizes ascending), restoring kpsi before every trial, and keeps the faster
+// one.  The choice is remembered per (N, GPU model) for ensembles created
+// later in the process.  vars_out[i], ms_out[2i], ms_out[2i+1]: the chosen
+// shape and the ms/step of shapes 0 and 1 for the i-th distinct size
+// (ascending; -1 where a size has one shape).
+int vk_ens_autotune_shapes(vk_ens* e, int nsteps, int* vars_out, double* ms_out)
+{
+    if (!e || nsteps <= 0) return fail(VK_ERR_ARG, "ens_autotune_shapes: bad arguments");
+    if (e->mode == 2) return fail(VK_ERR_STATE, "ens_autotune_shapes: the per-member mode has one shape");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<cplx*> save;
+    for (auto h : e->h) {
+        cplx* p = nullptr;
+        HIPCHK(hipMalloc(&p, f2(h) * sizeof(cplx)));
+        HIPCHK(hipMemcpy(p, h->K0, f2(h) * sizeof(cplx), hipMemcpyDeviceToDevice));
+        save.push_back(p);
+    }
+    auto restore = [&]() -> int {
+        for (size_t i = 0; i < e->h.size(); ++i) {
+            vk_gpe* h = e->h[i];
+            HIPCHK(hipMemcpy(h->K0, save[i], f2(h) * sizeof(cplx), hipMemcpyDeviceToDevice));
+            h->hstate = 0;
+        }
+        e->synced = 0;
+        return 0;
+    };
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int rc = 0;
+    for (size_t i = 0; i < e->sizes.size() && !rc; ++i) {
+        const int N = e->sizes[i];
+        if (vars_out) vars_out[i] = 0;
+        if (ms_out) ms_out[2 * i] = ms_out[2 * i + 1] = -1.0;
+        if (!vk::has_variant(N, 1)) continue;
+        double t[2] = {0.0, 0.0};
+        for (int v = 0; v <= 1 && !rc; ++v) {
+            set_member_variant(e, N, v);
+            if ((rc = ens_rebuild(e)) || (rc = restore()) || (rc = ens_steps_async(e, 1))) break;
+            if (hipEventRecord(a, e->stream) != hipSuccess) { rc = fail(VK_ERR_HIP, "event"); break; }
+            if ((rc = ens_steps_async(e, nsteps))) break;
+            if (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+                rc = fail(VK_ERR_HIP, "event");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            t[v] = ms / nsteps;
+        }
+        if (rc) break;
+        const int best = t[1] < t[0] ? 1 : 0;
+        set_member_variant(e, N, best);
+        if ((rc = ens_rebuild(e))) break;
+        if (vars_out) vars_out[i] = best;
+        if (ms_out) { ms_out[2 * i] = t[0]; ms_out[2 * i + 1] = t[1]; }
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        g_shape_cache[{N, gpu_model()}] = best;
+    }
+    if (!rc) rc = restore();
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (auto p : save) hipFree(p);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return rc;
 }
 
 }  // extern "C"

@@ -447,6 +447,7 @@ class Ensemble:
         n = len(params_list)
         arr = (nat.Params * n)(*params_list)
         self.params = list(params_list)
+        self.sizes = [p.N for p in params_list]
         h = nat.ctypes.c_void_p()
         nat.check(nat.lib().vk_ens_create(nat.ctypes.byref(h), n, arr))
         self.h = h
@@ -513,6 +514,22 @@ class Ensemble:
         ms = np.zeros(6)
         nat.check(nat.lib().vk_ens_autotune(self.h, int(nsteps), nat.ptr(ms)))
         return ms.reshape(3, 2)
+
+    def set_shape(self, N, var):
+        nat.check(nat.lib().vk_ens_set_shape(self.h, int(N), int(var)))
+
+    def shape(self, N):
+        v = nat.ctypes.c_int()
+        nat.check(nat.lib().vk_ens_get_shape(self.h, int(N), nat.ctypes.byref(v)))
+        return v.value
+
+    def autotune_shapes(self, nsteps=3):
+        """Times both member shapes per size; returns {N: (chosen, ms0, ms1)}."""
+        sizes = sorted(set(self.sizes))
+        vs = np.zeros(len(sizes), dtype=np.int32)
+        ms = np.zeros(2 * len(sizes))
+        nat.check(nat.lib().vk_ens_autotune_shapes(self.h, int(nsteps), vs.ctypes.data_as(nat.ip), nat.ptr(ms)))
+        return {N: (int(vs[i]), float(ms[2 * i]), float(ms[2 * i + 1])) for i, N in enumerate(sizes)}
 
 
 def ic_vortices_host(N, xs, ys, qs, xi, nimg=3):

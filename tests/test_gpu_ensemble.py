@@ -138,3 +138,73 @@ def test_ensemble_384_mix(G):
         n1 = e.diagnostics(r)[0][4]
         assert abs(n1 - nn) < 1e-11 * nn, r
     e.close()
+
+
+def test_ensemble_shapes(G, oracle_mod):
+    """Members of sizes 256..1024 in their second shape (8 points per
+    thread, radix-8 stages, half-width panels): within roundoff of the
+    single-field runs and within 1e-11 of the oracle; members left in
+    shape 0 stay bit-identical; reading a member between steps and the
+    per-member launch mode keep the state consistent; the shape autotuner
+    restores the state and reports both timings."""
+    say("case ensemble shapes")
+    sizes, ps, ics = members(G)
+    ref = single_runs(G, ps, ics, 5)
+    e = G.Ensemble(ps)
+    for r, (xs, ys, qs) in enumerate(ics):
+        e.set_ic_vortices(r, xs, ys, qs)
+    for N in (256, 512, 1024):
+        e.set_shape(N, 1)
+        assert e.shape(N) == 1
+    e.set_mode(0, True)
+    e.rk4(2)
+    _ = e.psi(3)              # a shape-1 member read between steps
+    e.set_mode(2, False)
+    e.rk4(1)                  # per-member launches use the single-field layout
+    e.set_mode(0, True)
+    e.rk4(2)
+    for r, N in enumerate(sizes):
+        a = e.psi(r)
+        if N == 128:
+            assert np.array_equal(a, ref[r]), r
+        else:
+            err = rel_l2(a, ref[r])
+            say("  member %d N=%d shape 1: rel-L2 vs single %.2e" % (r, N, err))
+            assert err < 1e-13, r
+    O = oracle_mod
+    r = 5                     # N = 1024
+    st = O.State(O.make_params(sizes[r], 1e-3, 2.5))
+    xs, ys, qs = ics[r]
+    g = G.GPE(ps[r])
+    g.set_ic_vortices(xs, ys, qs)
+    st.set_psi(g.psi, sync=True)
+    g.close()
+    st.rk4(5)
+    assert rel_l2(e.psi(r), st.psi) < 1e-11
+    e.close()
+    # autotuner: state restored, timings reported
+    e = G.Ensemble(ps)
+    for r, (xs, ys, qs) in enumerate(ics):
+        e.set_ic_vortices(r, xs, ys, qs)
+    res = e.autotune_shapes(2)
+    say("  autotune_shapes", res)
+    assert res[128][0] == 0 and res[128][1] == -1.0
+    for N in (256, 512, 1024):
+        assert res[N][1] > 0 and res[N][2] > 0 and e.shape(N) == res[N][0]
+    e.rk4(5)
+    for r, N in enumerate(sizes):
+        a = e.psi(r)
+        if e.shape(N) == 0:
+            assert np.array_equal(a, ref[r]), r
+        else:
+            assert rel_l2(a, ref[r]) < 1e-13, r
+    e.close()
+    # the tuned shapes are remembered for later ensembles until forgotten
+    e = G.Ensemble(ps)
+    assert all(e.shape(N) == res[N][0] for N in (256, 512, 1024))
+    e.close()
+    from quantumvortices_amd import _native as nat
+    nat.lib().vk_ens_forget_shapes()
+    e = G.Ensemble(ps)
+    assert all(e.shape(N) == 0 for N in (256, 512, 1024))
+    e.close()
