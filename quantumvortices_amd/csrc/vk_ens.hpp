@@ -1,7 +1,9 @@
 // Batched ensemble of independent realizations of unequal size.  One launch
 // per pass serves every realization: each workgroup finds its realization
 // through a block map and runs the size's pass body.  Sizes 128..1024 all
-// use 256-thread workgroups, so one kernel holds every size.
+// use 256-thread workgroups, so one kernel holds every size.  P1 stores the
+// x-space psi rows it computes (Xs) and P3 reads them, which saves P3 one of
+// its three transforms; the rows are the bits P3 would compute itself.
 #pragma once
 #include "vk_kernels.hpp"
 
@@ -11,7 +13,7 @@ struct EnsReal {
     int N;
     int var;        // shape variant (CfgE<N, var>)
     Geo g;
-    cplx *K0, *K1, *K2, *Hpsi, *Hrho, *HN;
+    cplx *K0, *K1, *K2, *Hpsi, *Hrho, *HN, *Xs;
     const cplx* tw;
     double alpha, omega, gsc, sc, dt;
     cplx pref;
@@ -38,7 +40,7 @@ __global__ void __launch_bounds__(256) k_ens_p1(const EnsReal* __restrict__ R, c
                                                 const int* __restrict__ off)
 {
     VK_ENS_SETUP
-    VK_ENS_SWITCH(e.N, (p1_body<C>(lb, nb, e.Hpsi, e.Hrho, e.tw, e.g, 1, e.sc)))
+    VK_ENS_SWITCH(e.N, (p1_body<C, true>(lb, nb, e.Hpsi, e.Hrho, e.tw, e.g, 1, e.sc, e.Xs)))
 }
 
 __global__ void __launch_bounds__(256) k_ens_p2(const EnsReal* __restrict__ R, const int* __restrict__ map,
@@ -53,8 +55,8 @@ __global__ void __launch_bounds__(256) k_ens_p3(const EnsReal* __restrict__ R, c
 {
     VK_ENS_SETUP
     const P3Part pa{};
-    VK_ENS_SWITCH(e.N, (p3_body<C, 0>(lb, nb, e.Hrho, e.Hrho, e.Hpsi, e.HN, e.tw, e.g, 1, 1, e.gsc, e.sc, e.sc,
-                                      1.0, pa)))
+    VK_ENS_SWITCH(e.N, (p3_body<C, 0, true>(lb, nb, e.Hrho, e.Hrho, nullptr, e.HN, e.tw, e.g, 0, 1, e.gsc, e.sc,
+                                            1.0, 1.0, pa, e.Xs)))
 }
 
 __global__ void __launch_bounds__(256) k_ens_p4(const EnsReal* __restrict__ R, const int* __restrict__ map,

@@ -917,7 +917,10 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         h->tr1_p1 = t1 && t1[0] == '1';
         h->tr1_p3 = t3 && t3[0] == '1';
     }
-    if (h->xpsi && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
+    // psi rows: the single-field RK4/ARGLE path at N >= VK_XPSI_MIN_N and the
+    // ensemble kernels (which also step single fields of N <= 1024)
+    const bool rows = h->xpsi || (N >= 128 && N <= 1024);
+    if (rows && hipMalloc(&h->Xs, fb) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc failed (psi rows)"));
     if (hipMalloc(&h->tw, sizeof(cplx) * N) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc tw"));
     // twiddles w_k = exp(-2 pi i k/N): long-double sin/cos on the first
     // octant, exact quadrant/octant symmetries elsewhere
@@ -1540,11 +1543,17 @@ double vk_gpe_pass_bytes(vk_gpe* h, int kind)
         for (int ky = -h->N / 2; ky < h->N / 2; ++ky)
             if (!masked_host(kx, ky, h->N)) ++disk;
     const double D = 16.0 * (double)disk;
+    // the RK4 step of a single field of N <= 1024 without particles runs the
+    // ensemble kernels, which pass psi rows from P1 to P3 as well
+    const bool rows = h->xpsi || (h->e1R && !(h->have_part && h->p.np > 0));
     switch (kind) {
-    case PK_P1: return F * (fr + fp + (h->xpsi ? 1.0 : 0.0));   // + psi rows written for P3
+    case PK_P1: return F * (fr + fp + (rows ? 1.0 : 0.0));   // + psi rows written for P3
     case PK_P2: return F * 2.0 * fp;
-    case PK_P3: return F * (fp + (h->xpsi ? 1.0 : fr) + fr);    // psi rows instead of Hpsi
-    case PK_P4: return F * (fr + fr) + D * (2 + 4 + 4 + 4) / 4.0;   // K traffic per stage
+    case PK_P3: return F * (fp + (rows ? 1.0 : fr) + fr);    // psi rows instead of Hpsi
+    // K accesses per mode, stages 1-4: K0 read + K2 write; K0, K2 read + K1,
+    // K2 write; K0, K2 read + K2 write (str2 is not read in stage 3); K0, K1,
+    // K2 read + K0 write
+    case PK_P4: return F * (fr + fr) + D * (2 + 4 + 3 + 4) / 4.0;
     default: return 0.0;
     }
 }
@@ -1609,7 +1618,7 @@ template <class C> void ens_lds(size_t* l)
 {
     l[0] = std::max(l[0], (size_t)C::TR * C::ROWRS * sizeof(double));
     l[1] = std::max(l[1], (size_t)C::TC * C::COLRS * sizeof(double));
-    l[2] = std::max(l[2], (size_t)C::TR * C::ROWRS * sizeof(double) + (size_t)C::S * C::ROWT * sizeof(double) + 16);
+    l[2] = std::max(l[2], (size_t)C::TR * C::ROWRS * sizeof(double) + 16);   // f in registers (psi rows)
     l[3] = std::max(l[3], (size_t)C::TC * C::COLRS * sizeof(double));
 }
 
@@ -1657,7 +1666,7 @@ vk::EnsReal ens_entry(const vk_gpe* h)
     x.N = h->N;
     x.var = h->ens_var;
     x.g = h->ens_var ? h->geo_v : h->geo;
-    x.K0 = h->K0; x.K1 = h->K1; x.K2 = h->K2; x.Hpsi = h->Hpsi; x.Hrho = h->Hrho; x.HN = h->HN;
+    x.K0 = h->K0; x.K1 = h->K1; x.K2 = h->K2; x.Hpsi = h->Hpsi; x.Hrho = h->Hrho; x.HN = h->HN; x.Xs = h->Xs;
     x.tw = h->tw;
     x.alpha = h->p.alpha;
     x.omega = h->p.omega;
