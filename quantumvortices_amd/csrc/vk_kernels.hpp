@@ -6,11 +6,22 @@
 #pragma once
 #include "vk_cfg.hpp"
 
-// optional occupancy request for the pass kernels (waves per SIMD)
+// optional occupancy request for the pass kernels (waves per SIMD), and for
+// the P2/P4 column passes alone
 #ifdef VK_WAVES
 #define VK_OCC __attribute__((amdgpu_waves_per_eu(VK_WAVES)))
 #else
 #define VK_OCC
+#endif
+#ifdef VK_COL_WAVES
+#define VK_COCC __attribute__((amdgpu_waves_per_eu(VK_COL_WAVES)))
+#else
+#define VK_COCC VK_OCC
+#endif
+#ifdef VK_P4_WAVES
+#define VK_P4OCC __attribute__((amdgpu_waves_per_eu(VK_P4_WAVES)))
+#else
+#define VK_P4OCC VK_COCC
 #endif
 
 namespace vk {
@@ -423,7 +434,7 @@ __device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ 
 }
 
 template <class C, int G = C::TC>
-__global__ void __launch_bounds__(G * C::T) VK_OCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
+__global__ void __launch_bounds__(G * C::T) VK_COCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
@@ -852,7 +863,7 @@ __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restr
 }
 
 template <class C, int STAGE>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+__global__ void __launch_bounds__(C::COLT) VK_P4OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
                                                 cplx* __restrict__ K2, const cplx* __restrict__ tw,
                                                 Geo g, RKArgs a)
