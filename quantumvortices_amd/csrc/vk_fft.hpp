@@ -6,8 +6,8 @@
 // length Ns reads the butterfly j = t + T*a from registers v[a + (S/R)*r],
 // applies twiddles w_{Ns*R}^{r*(j%Ns)}, does the radix-R DFT and writes to
 // position (j/Ns)*Ns*R + j%Ns + r*Ns.  Between stages the points go through
-// LDS (padded slots, XMap below, against bank conflicts of the strided
-// writes); the last stage's outputs land back in the same
+// LDS (padded slots p + p/16, lslot below, against bank conflicts of the
+// strided writes); the last stage's outputs land back in the same
 // register slots, so a global load feeds the first stage directly and the
 // last stage feeds a global store directly.  N = R^k needs k-1 exchanges.
 //
@@ -163,59 +163,33 @@ __host__ __device__ constexpr int radix_at(int N, int S, int st)
     return r;
 }
 
-// LDS slot of point p in the exchange of an N-point transform:
-//   p + p/16 + E*(p/256).
-// The workgroup's transforms are interleaved across the lanes of a wave
-// (two per wave in the 4096-point passes), and their regions lie a stride
-// of 16 (mod 64) doubles apart.  Measured on gfx950 with rocprofv3
-// (SQ_LDS_BANK_CONFLICT), the gathers t + T*s of such a pair are then
-// conflict-free with E = 0, but the 4096-point scatters are not; the extra
-// 8 slots per 256 (E = 8) make both scatters (sub-lengths 1 and 16) and the
-// gathers conflict-free.  Other sizes keep E = 0.
-#ifndef VK_XMAP_E
-#define VK_XMAP_E 8
-#endif
-// keep the scatter's 64-bit LDS writes apart (no ds_write2_b64 pairs)
-#ifndef VK_LDS_SEP
-#define VK_LDS_SEP 0
-#endif
-template <int N> struct XMap {
-    static constexpr int E = N == 4096 ? VK_XMAP_E : 0;
-    __host__ __device__ static constexpr int idx(int p) { return p + (p >> 4) + E * (p >> 8); }
-};
+// LDS slot of point p in an exchange: one pad double per 16 points, so that
+// the 16-strided writes of a stage with sub-length 1 spread over the banks
+__device__ __forceinline__ int lslot(int p) { return p + (p >> 4); }
 
 // LDS words per transform for one real component
 template <int N> struct Lds {
-    static constexpr int words = N + N / 16 + XMap<N>::E * (N / 256);
+    static constexpr int words = N + N / 16;
 };
 
-// XMap<N>::idx(t + T*s), written so that the compiler sees one base address
-// and a constant offset per s where that is exact (one address register for
-// all S reads): 16 | T, and 256 | T when E != 0
-template <int N, int T> __device__ __forceinline__ int gather_idx(int t, int s)
+// lslot(t + T*s), written so that the compiler sees one base address and a
+// constant offset per s where that is exact (16 | T): one address register
+// for all S reads
+template <int T> __device__ __forceinline__ int gather_idx(int t, int s)
 {
-    using M = XMap<N>;
-    if constexpr (T % 16 == 0 && (M::E == 0 || T % 256 == 0))
-        return M::idx(t) + s * (T + T / 16 + M::E * (T / 256));
-    else
-        return M::idx(t + T * s);
+    if constexpr (T % 16 == 0) return lslot(t) + s * (T + T / 16);
+    else return lslot(t + T * s);
 }
 
-// XMap<N>::idx(idxD + r*NS) for the scatter of a stage with sub-length NS
-// and radix R, again as base + constant offset where that is exact:
-//   NS = 1, R | 16: idxD = R*j, and R*j + r crosses no multiple of 16 (nor
-//     of 256) for r < R;
-//   NS a multiple of 16 with NS*R <= 256 (E != 0) or any (E = 0): the term
-//     p/16 grows by NS/16 per r and p/256 stays that of idxD.
-template <int N, int NS, int R> __device__ __forceinline__ int scatter_idx(int idxD, int r)
+// lslot(idxD + r*NS) for the scatter of a stage with sub-length NS and
+// radix R, again as base + constant offset where that is exact: NS a
+// multiple of 16, or NS = 1 with R dividing 16 (idxD = R*j, and R*j + r
+// crosses no multiple of 16 for r < R)
+template <int NS, int R> __device__ __forceinline__ int scatter_idx(int idxD, int r)
 {
-    using M = XMap<N>;
-    if constexpr (NS == 1 && 16 % R == 0)
-        return M::idx(idxD) + r;
-    else if constexpr (NS % 16 == 0 && (M::E == 0 || NS * R <= 256))
-        return M::idx(idxD) + r * (NS + NS / 16);
-    else
-        return M::idx(idxD + r * NS);
+    if constexpr (NS % 16 == 0) return lslot(idxD) + r * (NS + NS / 16);
+    else if constexpr (NS == 1 && 16 % R == 0) return lslot(idxD) + r;
+    else return lslot(idxD + r * NS);
 }
 
 // One Stockham stage, then recurse.  v: registers; re/im: this transform's
@@ -273,28 +247,22 @@ struct Stage {
                     const int j = t + T * a;
                     const int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        re[scatter_idx<N, NS, R>(idxD, r)] = v[a + (S / R) * r].x;
-                        if constexpr (VK_LDS_SEP != 0) asm volatile("" ::: "memory");
-                    }
+                    for (int r = 0; r < R; ++r) re[scatter_idx<NS, R>(idxD, r)] = v[a + (S / R) * r].x;
                 }
                 lds_barrier();
 #pragma unroll
-                for (int s = 0; s < S; ++s) v[s].x = re[gather_idx<N, T>(t, s)];
+                for (int s = 0; s < S; ++s) v[s].x = re[gather_idx<T>(t, s)];
                 lds_barrier();
 #pragma unroll
                 for (int a = 0; a < S / R; ++a) {
                     const int j = t + T * a;
                     const int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        re[scatter_idx<N, NS, R>(idxD, r)] = v[a + (S / R) * r].y;
-                        if constexpr (VK_LDS_SEP != 0) asm volatile("" ::: "memory");
-                    }
+                    for (int r = 0; r < R; ++r) re[scatter_idx<NS, R>(idxD, r)] = v[a + (S / R) * r].y;
                 }
                 lds_barrier();
 #pragma unroll
-                for (int s = 0; s < S; ++s) v[s].y = re[gather_idx<N, T>(t, s)];
+                for (int s = 0; s < S; ++s) v[s].y = re[gather_idx<T>(t, s)];
                 lds_barrier();
             } else {
 #pragma unroll
@@ -303,13 +271,13 @@ struct Stage {
                     const int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        re[scatter_idx<N, NS, R>(idxD, r)] = v[a + (S / R) * r].x;
-                        im[scatter_idx<N, NS, R>(idxD, r)] = v[a + (S / R) * r].y;
+                        re[scatter_idx<NS, R>(idxD, r)] = v[a + (S / R) * r].x;
+                        im[scatter_idx<NS, R>(idxD, r)] = v[a + (S / R) * r].y;
                     }
                 }
                 lds_barrier();
 #pragma unroll
-                for (int s = 0; s < S; ++s) v[s] = mk(re[gather_idx<N, T>(t, s)], im[gather_idx<N, T>(t, s)]);
+                for (int s = 0; s < S; ++s) v[s] = mk(re[gather_idx<T>(t, s)], im[gather_idx<T>(t, s)]);
                 lds_barrier();
             }
             Stage<N, S, ST + 1, NS * R, SPLIT>::run(v, re, im, t, tw);

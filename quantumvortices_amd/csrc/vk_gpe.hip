@@ -274,7 +274,10 @@ void col_launch(vk_gpe* h, int kind, int nblocks, Kern k, A... args)
 // one column per workgroup (1366 in fuller waves) is 4% faster; at 2048^2
 // the same split is 10% slower (16-byte row pieces of 4-column panels).
 // Results are bitwise the same either way.
-template <class C> constexpr int p2_cols() { return C::N == 4096 ? 1 : C::TC; }
+#ifndef VK_P2_COLS_4096
+#define VK_P2_COLS_4096 1
+#endif
+template <class C> constexpr int p2_cols() { return C::N == 4096 ? VK_P2_COLS_4096 : C::TC; }
 
 // column pass with G columns per workgroup over npan panels
 template <class C, int G, class Kern, class... A>
@@ -521,7 +524,7 @@ int particle_forces(vk_gpe* h, const cplx* Krho, int nrows, int half, int stage)
     ev_begin(h);
     hipLaunchKernelGGL(vk::k_force_rows<PG>, dim3(nrows, (np + PG - 1) / PG), dim3(256), 0, h->stream, Krho,
                        (const cplx*)h->Vhat, (const cplx*)h->EY, h->fpart, np, h->N);
-    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, h->stream, (const double*)h->fpart, (const cplx*)h->EX,
+    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(vk::PSTEP_T), 0, h->stream, (const double*)h->fpart, (const cplx*)h->EX,
                        h->pst, part_args(h, nrows, half, stage));
     ev_end(h, PK_PART);
     return 0;
@@ -543,7 +546,7 @@ int particle_step_rs(vk_gpe* h, int nblk, int nw, int stage)
     const double coef = 2.0 * h->p.alpha * (dx * dx) / ((double)h->N * (double)h->N);
     hipLaunchKernelGGL(vk::k_pforce_rs, dim3(h->p.np), dim3(256), 0, st, (const double*)h->fpart,
                        (const unsigned char*)h->phit, nblk, nw, h->p.np, coef, h->pst + 28 * h->p.np + 8);
-    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(64), 0, st, (const double*)h->fpart, (const cplx*)h->EX,
+    hipLaunchKernelGGL(vk::k_pstep, dim3(1), dim3(vk::PSTEP_T), 0, st, (const double*)h->fpart, (const cplx*)h->EX,
                        h->pst, part_args(h, 0, 0, stage, 1, nblk));
     ev_end(h, PK_PART);
     if (side) hipEventRecord(h->ev_ps, h->pstream);

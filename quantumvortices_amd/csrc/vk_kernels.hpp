@@ -18,10 +18,26 @@
 #else
 #define VK_COCC VK_OCC
 #endif
-#ifdef VK_P4_WAVES
+// P4 at 4 waves per SIMD (<= 128 VGPRs: two 512-thread workgroups per CU)
+// is 9% faster than at the compiler's choice of 134-181 VGPRs
+#ifndef VK_P4_WAVES
+#define VK_P4_WAVES 4
+#endif
+#if VK_P4_WAVES > 0
 #define VK_P4OCC __attribute__((amdgpu_waves_per_eu(VK_P4_WAVES)))
 #else
 #define VK_P4OCC VK_COCC
+#endif
+#ifdef VK_P1_WAVES
+#define VK_P1OCC __attribute__((amdgpu_waves_per_eu(VK_P1_WAVES)))
+#else
+#define VK_P1OCC VK_OCC
+#endif
+// P3 on the psi-row path without particles (PM = 0) only
+#ifdef VK_P3_WAVES
+#define VK_P3OCC(PM) __attribute__((amdgpu_waves_per_eu((PM) == 0 ? VK_P3_WAVES : 1)))
+#else
+#define VK_P3OCC(PM) VK_OCC
 #endif
 
 namespace vk {
@@ -277,7 +293,7 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1(const cplx* __restrict__ 
 }
 
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p1x(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
+__global__ void __launch_bounds__(C::ROWT) VK_P1OCC k_p1x(const cplx* __restrict__ Hpsi, cplx* __restrict__ Hrho,
                                                  const cplx* __restrict__ tw, Geo g, int psi_pmode,
                                                  double scale, cplx* __restrict__ Xpsi)
 {
@@ -786,7 +802,7 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3a(const cplx* __restrict__
 }
 
 template <class C, int PM>
-__global__ void __launch_bounds__(C::ROWT) VK_OCC k_p3x(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
+__global__ void __launch_bounds__(C::ROWT) VK_P3OCC(PM) k_p3x(const cplx* __restrict__ Hrhod, const cplx* __restrict__ HV,
                                                  const cplx* __restrict__ Xpsi, cplx* __restrict__ HN,
                                                  const cplx* __restrict__ tw, Geo g, int store_pmode, double gs,
                                                  double vs, double vsign, P3Part pa)
@@ -1191,18 +1207,26 @@ struct PartArgs {
 };
 
 // Particle forces and one RK4 stage of gp_particle_motion_rk4
-// (src/gpe_particle.f90:219-324, 378-531), one workgroup of 64 lanes.
+// (src/gpe_particle.f90:219-324, 378-531), one workgroup of PSTEP_T threads.
 // Fluid force F_j = -2 alpha dvol Re sum_x d_j rho V_p
 //   = (2 alpha dvol/N^2) Im sum_k k_j rhohat_d(k) Vhat(-k) e^{i k.q_p}.
-// Particles advance one at a time (the short-range force on particle ip
-// sees the already advanced particles < ip, as in the reference).
+// The reference advances the particles one at a time, so the short-range
+// force on particle ip sees the particles q < ip at their new positions and
+// q > ip at their old ones.  A particle's new position depends only on the
+// state at the start of the stage (tpos, its velocity, the earlier stages'
+// kq), so all new positions are formed first; the forces then run one
+// particle per wave, each with the lane order and shuffle tree of the
+// one-wave form (lane l sums q = l, l + 64, ...), from positions in LDS.
 // st: pos(2np) vel(2np) fext(2np) tpos tvel ku[4](2np) kq[4](2np) frc(2np)
 //     then diag[5] = {eobj_kin, esrep, momx, momy, -}
-__global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict__ EX, double* __restrict__ st,
-                        PartArgs a)
+constexpr int PSTEP_T = 512;
+__global__ void __launch_bounds__(PSTEP_T) k_pstep(const double* __restrict__ fpart, const cplx* __restrict__ EX,
+                                                   double* __restrict__ st, PartArgs a)
 {
     __shared__ double fl[2 * 256];
-    __shared__ double red[2];
+    __shared__ double po[2 * 256];   // positions at the start of the stage
+    __shared__ double pn[2 * 256];   // positions after the stage
+    __shared__ double ff[2 * 256];   // total force
     const int np = a.np;
     double* pos = st;
     double* vel = st + 2 * np;
@@ -1213,14 +1237,15 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
     double* kq = st + 18 * np;   // [4][2np]
     double* frc = st + 26 * np;
     double* dg = st + 28 * np;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    constexpr int NW = PSTEP_T / 64;
     // fluid forces, fixed order over rows (spectral) or row blocks (real space)
-    for (int pp = lane; pp < np && a.src == 1; pp += 64) {
+    for (int pp = tid; pp < np && a.src == 1; pp += PSTEP_T) {
         // reduced by k_pforce_rs into st[28 np + 8 ...]
         fl[2 * pp] = st[28 * np + 8 + 2 * pp];
         fl[2 * pp + 1] = st[28 * np + 8 + 2 * pp + 1];
     }
-    for (int pp = lane; pp < np && a.src == 0; pp += 64) {
+    for (int pp = tid; pp < np && a.src == 0; pp += PSTEP_T) {
         double bx = 0.0, by = 0.0;
         for (int kx = 0; kx < a.nrows; ++kx) {
             const double* f = fpart + ((size_t)kx * np + pp) * 4;
@@ -1235,16 +1260,31 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
         fl[2 * pp] = a.coef * bx;
         fl[2 * pp + 1] = a.coef * by;
     }
+    // start-of-stage positions, and the new positions
+    // (stage 1 first saves tpos, tvel)
+    for (int i = tid; i < 2 * np; i += PSTEP_T) {
+        const double q = pos[i], u = vel[i];
+        if (a.stage == 1) { tpos[i] = q; tvel[i] = u; }
+        po[i] = q;
+        double qn = q;
+        if (a.stage > 0) {
+            const double kqv = a.dt * u;
+            const double tp = a.stage == 1 ? q : tpos[i];
+            if (a.stage <= 2) qn = tp + 0.5 * kqv;
+            else if (a.stage == 3) qn = tp + kqv;
+            else qn = tp + (1.0 / 6.0) * (kq[i] + 2.0 * kq[2 * np + i] + 2.0 * kq[4 * np + i] + kqv);
+        }
+        pn[i] = qn;
+    }
     __syncthreads();
-    if (a.stage == 1)
-        for (int i = lane; i < 2 * np; i += 64) { tpos[i] = pos[i]; tvel[i] = vel[i]; }
-    __syncthreads();
-    for (int ip = 0; ip < np; ++ip) {
+    // short-range forces, one particle per wave
+    for (int ip = wv; ip < np; ip += NW) {
         double sx = 0.0, sy = 0.0;
-        const double px = pos[2 * ip], py = pos[2 * ip + 1];
+        const double px = po[2 * ip], py = po[2 * ip + 1];
         for (int qq = lane; qq < np; qq += 64) {
             if (qq == ip) continue;
-            const double x = px - pos[2 * qq], y = py - pos[2 * qq + 1];
+            const double* P = (a.stage > 0 && qq < ip) ? pn : po;
+            const double x = px - P[2 * qq], y = py - P[2 * qq + 1];
             const double b = -3.0 + cos(x) + cos(y);
             const double b2 = b * b, b4 = b2 * b2;
             const double den = b4 * b2 * b;
@@ -1258,33 +1298,29 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
         }
         if (np == 1) { sx = 0.0; sy = 0.0; }
         if (lane == 0) {
-            const double F[2] = {fl[2 * ip] + sx + fext[2 * ip], fl[2 * ip + 1] + sy + fext[2 * ip + 1]};
-            frc[2 * ip] = F[0];
-            frc[2 * ip + 1] = F[1];
-            if (a.stage > 0) {
-                const int sidx = a.stage - 1;
-                for (int d = 0; d < 2; ++d) {
-                    const int i = 2 * ip + d;
-                    const double kuv = a.dt * F[d] / a.mass;
-                    const double kqv = a.dt * vel[i];
-                    ku[sidx * 2 * np + i] = kuv;
-                    kq[sidx * 2 * np + i] = kqv;
-                    if (a.stage <= 2) {
-                        vel[i] = tvel[i] + 0.5 * kuv;
-                        pos[i] = tpos[i] + 0.5 * kqv;
-                    } else if (a.stage == 3) {
-                        vel[i] = tvel[i] + kuv;
-                        pos[i] = tpos[i] + kqv;
-                    } else {
-                        vel[i] = tvel[i] + (1.0 / 6.0) * (ku[i] + 2.0 * ku[2 * np + i] + 2.0 * ku[4 * np + i] + ku[6 * np + i]);
-                        pos[i] = tpos[i] + (1.0 / 6.0) * (kq[i] + 2.0 * kq[2 * np + i] + 2.0 * kq[4 * np + i] + kq[6 * np + i]);
-                    }
-                }
-            }
+            ff[2 * ip] = fl[2 * ip] + sx + fext[2 * ip];
+            ff[2 * ip + 1] = fl[2 * ip + 1] + sy + fext[2 * ip + 1];
         }
-        __syncthreads();
     }
-    if (a.stage == 0) {
+    __syncthreads();
+    // stage update of every particle
+    for (int i = tid; i < 2 * np; i += PSTEP_T) {
+        const double F = ff[i];
+        frc[i] = F;
+        if (a.stage > 0) {
+            const int sidx = a.stage - 1;
+            const double kuv = a.dt * F / a.mass;
+            const double u = vel[i];
+            ku[sidx * 2 * np + i] = kuv;
+            kq[sidx * 2 * np + i] = a.dt * u;
+            const double tv = a.stage == 1 ? u : tvel[i];
+            if (a.stage <= 2) vel[i] = tv + 0.5 * kuv;
+            else if (a.stage == 3) vel[i] = tv + kuv;
+            else vel[i] = tv + (1.0 / 6.0) * (ku[i] + 2.0 * ku[2 * np + i] + 2.0 * ku[4 * np + i] + kuv);
+            pos[i] = pn[i];
+        }
+    }
+    if (a.stage == 0 && wv == 0) {
         // particle energies and momentum (src/diagnostics.f90:210-239)
         double ek = 0.0, es = 0.0, mx = 0.0, my = 0.0;
         for (int i1 = lane; i1 < np; i1 += 64) {
@@ -1293,7 +1329,7 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
             my += a.mass * vel[2 * i1 + 1];
             for (int i2 = 0; i2 < np; ++i2) {
                 if (i1 == i2) continue;
-                const double x = pos[2 * i1] - pos[2 * i2], y = pos[2 * i1 + 1] - pos[2 * i2 + 1];
+                const double x = po[2 * i1] - po[2 * i2], y = po[2 * i1 + 1] - po[2 * i2 + 1];
                 const double b = -3.0 + cos(x) + cos(y);
                 const double b2 = b * b;
                 es += a.efac / (b2 * b2 * b2);
@@ -1313,7 +1349,6 @@ __global__ void k_pstep(const double* __restrict__ fpart, const cplx* __restrict
             dg[3] = my;
         }
     }
-    (void)red;
 }
 
 // ----------------------------------------------------------------------------
