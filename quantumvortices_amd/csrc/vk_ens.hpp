@@ -7,6 +7,19 @@
 #pragma once
 #include "vk_kernels.hpp"
 
+// Occupancy of the ensemble passes (waves per SIMD; <= 168 VGPRs for 3):
+// three 256-thread workgroups per CU for the row passes (P1, P3) and the
+// column passes (P2, P4) make the 384-member step 4.7% faster than the
+// compiler's own register budget (two to three), and four are slower.
+#ifndef VK_ENS_ROW_WAVES
+#define VK_ENS_ROW_WAVES 3
+#endif
+#ifndef VK_ENS_COL_WAVES
+#define VK_ENS_COL_WAVES 3
+#endif
+#define VK_ENS_ROCC __attribute__((amdgpu_waves_per_eu(VK_ENS_ROW_WAVES)))
+#define VK_ENS_COCC __attribute__((amdgpu_waves_per_eu(VK_ENS_COL_WAVES)))
+
 namespace vk {
 
 struct EnsReal {
@@ -36,21 +49,21 @@ struct EnsReal {
     const EnsReal& e = R[r];                                              \
     const int lb = blockIdx.x - off[2 * r], nb = off[2 * r + 1] - off[2 * r];   // [start, end) of r
 
-__global__ void __launch_bounds__(256) k_ens_p1(const EnsReal* __restrict__ R, const int* __restrict__ map,
+__global__ void __launch_bounds__(256) VK_ENS_ROCC k_ens_p1(const EnsReal* __restrict__ R, const int* __restrict__ map,
                                                 const int* __restrict__ off)
 {
     VK_ENS_SETUP
     VK_ENS_SWITCH(e.N, (p1_body<C, true>(lb, nb, e.Hpsi, e.Hrho, e.tw, e.g, 1, e.sc, e.Xs)))
 }
 
-__global__ void __launch_bounds__(256) k_ens_p2(const EnsReal* __restrict__ R, const int* __restrict__ map,
+__global__ void __launch_bounds__(256) VK_ENS_COCC k_ens_p2(const EnsReal* __restrict__ R, const int* __restrict__ map,
                                                 const int* __restrict__ off)
 {
     VK_ENS_SETUP
     VK_ENS_SWITCH(e.N, (p2_body<C>(lb, nb, e.Hrho, e.Hrho, (const cplx*)e.K1, e.K1, e.tw, e.g, 0)))
 }
 
-__global__ void __launch_bounds__(256) k_ens_p3(const EnsReal* __restrict__ R, const int* __restrict__ map,
+__global__ void __launch_bounds__(256) VK_ENS_ROCC k_ens_p3(const EnsReal* __restrict__ R, const int* __restrict__ map,
                                                 const int* __restrict__ off)
 {
     VK_ENS_SETUP
@@ -59,7 +72,7 @@ __global__ void __launch_bounds__(256) k_ens_p3(const EnsReal* __restrict__ R, c
                                             1.0, 1.0, pa, e.Xs)))
 }
 
-__global__ void __launch_bounds__(256) k_ens_p4(const EnsReal* __restrict__ R, const int* __restrict__ map,
+__global__ void __launch_bounds__(256) VK_ENS_COCC k_ens_p4(const EnsReal* __restrict__ R, const int* __restrict__ map,
                                                 const int* __restrict__ off, int stage)
 {
     VK_ENS_SETUP

@@ -917,7 +917,10 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         h->p1pf = p1_prefetch_ok(h);
         const char* t1 = std::getenv("VK_P1_TR1");
         const char* t3 = std::getenv("VK_P3_TR1");
-        h->tr1_p1 = t1 && t1[0] == '1';
+        // P1 with one row per workgroup (three 256-thread workgroups per CU)
+        // is 6% faster at 4096^2 and the default there; P3 is faster with two
+        // rows.  VK_P1_TR1 / VK_P3_TR1 = 0 or 1 override.
+        h->tr1_p1 = t1 ? t1[0] == '1' : N == 4096;
         h->tr1_p3 = t3 && t3[0] == '1';
     }
     // psi rows: the single-field RK4/ARGLE path at N >= VK_XPSI_MIN_N and the

@@ -4,8 +4,9 @@ Field: 256 random-sign vortices (seed 3), the first 4 of the workload's
 seeded particle positions (seed 4), V0 = 10 g, d = 2 xi (the real-space
 potential path inside the nonlinear row pass), mass 5x the displaced fluid.
 Two RK4 steps on the GPU, with the persistent prefetching P1 and with the
-plain P1 (VK_P1_PREFETCH=1 / 0), and with one row per workgroup in the row
-passes (VK_P1_TR1, VK_P3_TR1), each against the same two oracle steps
+plain P1 (VK_P1_PREFETCH=1 / 0), and with the default row-pass shapes (one
+row per workgroup in P1, two in P3), two rows in both, or one row in both
+(VK_P1_TR1, VK_P3_TR1), each against the same two oracle steps
 (src/gpe.f90:310-494, src/gpe_particle.f90:186-531), then the diagnostics
 (energies, six spectra, particle forces and particle energies,
 src/diagnostics.f90:54-533).  The oracle runs on the host's cores."""
@@ -61,13 +62,16 @@ def oracle_run(cfg, oracle_mod):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("prefetch,tr1", [("0", "0"), ("1", "0"), ("0", "1")])
+@pytest.mark.parametrize("prefetch,tr1", [("0", None), ("0", "0"), ("1", "0"), ("0", "1")])
 def test_cfg4096_particles_vs_oracle(cfg, oracle_run, monkeypatch, prefetch, tr1):
     W = cfg
     R = oracle_run
     monkeypatch.setenv("VK_P1_PREFETCH", prefetch)
-    monkeypatch.setenv("VK_P1_TR1", tr1)
-    monkeypatch.setenv("VK_P3_TR1", tr1)
+    for k in ("VK_P1_TR1", "VK_P3_TR1"):
+        if tr1 is None:
+            monkeypatch.delenv(k, raising=False)     # the default row-pass shapes
+        else:
+            monkeypatch.setenv(k, tr1)
     say("case cfg4096 GPU, VK_P1_PREFETCH=%s VK_P1_TR1=VK_P3_TR1=%s" % (prefetch, tr1))
     g, _ = W.particle_workload(4096, 2.5e-4, npart=NPART, mass=R["mass"])
     g.gpe_rk4(STEPS)
