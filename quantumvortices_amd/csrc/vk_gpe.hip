@@ -254,7 +254,7 @@ void row_launch(vk_gpe* h, int kind, Kern k, A... args)
 // particle positions and a one-bit-per-particle hit mask
 template <class C> size_t p3_extra(const vk_gpe* h)
 {
-    const bool park = !h->xpsi || (h->have_part && h->p.np > 0 && h->rs_part);
+    const bool park = !h->xpsi || (h->have_part && h->p.np > 0 && h->rs_part && !vk::kP3FregPart);
     const size_t np = (size_t)std::max(h->p.np, 1);
     return (park ? (size_t)C::S * C::ROWT * sizeof(double) : 0) + 2 * np * sizeof(double);
 }

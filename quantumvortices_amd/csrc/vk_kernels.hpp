@@ -33,8 +33,18 @@
 #else
 #define VK_P1OCC VK_OCC
 #endif
-// P3 on the psi-row path without particles (PM = 0) only
-#ifdef VK_P3_WAVES
+// P3 with real-space particles keeps its factor f in registers too, not in
+// 64 KB of LDS slots (P3 0.186 -> 0.184 ms; results bitwise the same)
+#ifndef VK_P3_FREG_PART
+#define VK_P3_FREG_PART 1
+#endif
+namespace vk { constexpr bool kP3FregPart = VK_P3_FREG_PART != 0; }
+// P3 on the psi-row path without particles (PM = 0) at 4 waves per SIMD
+// (138 -> 128 VGPRs, two workgroups per CU): +0.8% pure-GPE steps/s
+#ifndef VK_P3_WAVES
+#define VK_P3_WAVES 4
+#endif
+#if VK_P3_WAVES > 0
 #define VK_P3OCC(PM) __attribute__((amdgpu_waves_per_eu((PM) == 0 ? VK_P3_WAVES : 1)))
 #else
 #define VK_P3OCC(PM) VK_OCC
@@ -568,10 +578,11 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
-    // f in registers (XPSI without real-space particles) or parked in LDS
-    // slots after the transform regions; particle positions staged after
-    // those during the first transform, read back as broadcasts
-    constexpr bool FREG = XPSI && (flags & 2) == 0;
+    // f in registers on the psi-row path (with real-space particles too,
+    // unless VK_P3_FREG_PART=0), else parked in LDS slots after the
+    // transform regions; particle positions staged after those during the
+    // first transform, read back as broadcasts
+    constexpr bool FREG = XPSI && ((flags & 2) == 0 || kP3FregPart);
     double* qlds = lds + TR * C::ROWRS + (FREG ? 0 : S * C::ROWT);
     if constexpr ((flags & 2) != 0) {
         for (int i = threadIdx.x; i < 2 * pa.np; i += blockDim.x) qlds[i] = pa.pos[i];
@@ -773,10 +784,14 @@ __device__ __forceinline__ void p3_body(int blk_, int nblk_, const cplx* __restr
         }
     }
     fft_fwd<N, S, C::SPLIT>(v, lg, t, tw);
+    // branch-free buffer stores (an unstored panel gets the out-of-range
+    // offset and is dropped): conditional stores made the compiler sink the
+    // last butterfly into one branch per store (P3 0.191 -> 0.187 ms)
+    const brsrc rN = mkbuf(HN, N);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int pp = VK_PANEL(s);
-        if (panel_stored(pp, NP, g, store_pmode)) HN[hoff<C>(pp, y, c)] = v[s];
+        bst(rN, boff(hoff<C>(pp, y, c), panel_stored(pp, NP, g, store_pmode)), v[s]);
     }
 }
 
