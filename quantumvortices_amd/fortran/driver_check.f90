@@ -4,7 +4,9 @@
 ! writes psi, kpsi and the energies to a stream file (argument 1) for
 ! comparison, the diagnostics' text files into argument 2; then
 ! takes Ntt = 32 spatio-temporal snapshots, one per RK4 step, and appends
-! spec_sts2d and arr1.
+! spec_sts2d and arr1; then transforms psi with the work-array plans of
+! fft() ('gpnlin' forward and back, 'cmpxtmp' forward on psi**2) and
+! appends psi, the transforms and the round trip.
 PROGRAM driver_check
   USE ISO_C_BINDING
   USE vikshobha_c
@@ -51,6 +53,16 @@ PROGRAM driver_check
   END DO
   CALL comp_sts()
   WRITE(u) spec_sts2d, arr1
+  ! the work-array plans of fft(): gpnlin forward and back, cmpxtmp forward
+  CALL sync_host()
+  cgp_phys_nlin = psi
+  CALL fft('gpnlin', forward)
+  WRITE(u) psi, cgp_four_nlin
+  CALL fft('gpnlin', inverse)
+  WRITE(u) cgp_phys_nlin
+  phys_cmpxtmp = psi * psi
+  CALL fft('cmpxtmp', forward)
+  WRITE(u) four_cmpxtmp
   CLOSE(u)
   WRITE(*,'(A,ES22.14,A,ES22.14)') 'driver_check: norm=', norm, ' etot=', etot
   CALL clear_fft()

@@ -63,6 +63,12 @@ MODULE vikshobha_c
       TYPE(C_PTR), VALUE :: h
       COMPLEX(C_DOUBLE_COMPLEX) :: kpsi(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION vk_fft2d(h, dir, xin, xout) BIND(C, NAME='vk_fft2d')
+      IMPORT :: C_PTR, C_INT
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: dir
+      TYPE(C_PTR), VALUE :: xin, xout
+    END FUNCTION
     INTEGER(C_INT) FUNCTION vk_gpe_get_psi(h, psi, kpsi) BIND(C, NAME='vk_gpe_get_psi')
       IMPORT :: C_INT, C_PTR, C_DOUBLE_COMPLEX
       TYPE(C_PTR), VALUE :: h
@@ -197,6 +203,16 @@ MODULE vikshobha_gpu
   TYPE(vk_gpe_params) :: params
   TYPE(C_PTR) :: vkh = C_NULL_PTR
   COMPLEX(C_DOUBLE_COMPLEX), ALLOCATABLE, TARGET :: psi(:,:), kpsi(:,:)
+  ! the reference's other transform work arrays (src/mod_fft.f90:77-118):
+  ! each plan's physical (Nx,Ny) and Fourier (Ny,Nx, transposed) arrays share
+  ! one buffer, as FFTW's in-place plans there do
+  COMPLEX(C_DOUBLE_COMPLEX), ALLOCATABLE, TARGET :: work_store(:,:)
+  COMPLEX(C_DOUBLE_COMPLEX), POINTER :: cgp_phys_nlin(:,:) => NULL(), cgp_four_nlin(:,:) => NULL()
+  COMPLEX(C_DOUBLE_COMPLEX), POINTER :: phys_cmpxtmp(:,:) => NULL(), four_cmpxtmp(:,:) => NULL()
+  COMPLEX(C_DOUBLE_COMPLEX), POINTER :: cgp_phys_tmp1(:,:) => NULL(), cgp_four_tmp1(:,:) => NULL()
+  COMPLEX(C_DOUBLE_COMPLEX), POINTER :: cgp_phys_tmp2(:,:) => NULL(), cgp_four_tmp2(:,:) => NULL()
+  COMPLEX(C_DOUBLE_COMPLEX), POINTER :: cgp_phys_tmp3(:,:) => NULL(), cgp_four_tmp3(:,:) => NULL()
+  COMPLEX(C_DOUBLE_COMPLEX), POINTER :: cgp_phys_trans_vobj(:,:) => NULL(), cgp_four_trans_vobj(:,:) => NULL()
   REAL(C_DOUBLE) :: egrad, echem, eintr, etot, norm, momx, momy, n0, argle_residual
   REAL(C_DOUBLE), ALLOCATABLE :: spec_gp(:,:)
   ! checkpoint layout: slabs written per file (nsplit_current) and slabs the
@@ -290,6 +306,20 @@ CONTAINS
     nshell = vk_nshell(INT(Nx, C_INT))
     ALLOCATE(psi(Nx, Ny), kpsi(Ny, Nx), spec_gp(0:nshell, 1:6))
     psi = (0.0_C_DOUBLE, 0.0_C_DOUBLE); kpsi = psi
+    ALLOCATE(work_store(Nx * Ny, 6))
+    work_store = (0.0_C_DOUBLE, 0.0_C_DOUBLE)
+    CALL C_F_POINTER(C_LOC(work_store(1, 1)), cgp_phys_nlin, [Nx, Ny])
+    CALL C_F_POINTER(C_LOC(work_store(1, 1)), cgp_four_nlin, [Ny, Nx])
+    CALL C_F_POINTER(C_LOC(work_store(1, 2)), phys_cmpxtmp, [Nx, Ny])
+    CALL C_F_POINTER(C_LOC(work_store(1, 2)), four_cmpxtmp, [Ny, Nx])
+    CALL C_F_POINTER(C_LOC(work_store(1, 3)), cgp_phys_tmp1, [Nx, Ny])
+    CALL C_F_POINTER(C_LOC(work_store(1, 3)), cgp_four_tmp1, [Ny, Nx])
+    CALL C_F_POINTER(C_LOC(work_store(1, 4)), cgp_phys_tmp2, [Nx, Ny])
+    CALL C_F_POINTER(C_LOC(work_store(1, 4)), cgp_four_tmp2, [Ny, Nx])
+    CALL C_F_POINTER(C_LOC(work_store(1, 5)), cgp_phys_tmp3, [Nx, Ny])
+    CALL C_F_POINTER(C_LOC(work_store(1, 5)), cgp_four_tmp3, [Ny, Nx])
+    CALL C_F_POINTER(C_LOC(work_store(1, 6)), cgp_phys_trans_vobj, [Nx, Ny])
+    CALL C_F_POINTER(C_LOC(work_store(1, 6)), cgp_four_trans_vobj, [Ny, Nx])
   END SUBROUTINE
 
   FUNCTION fnrst() RESULT(s)
@@ -298,21 +328,38 @@ CONTAINS
     s = ADJUSTL(s)
   END FUNCTION
 
-  ! fft(plan_name, direction) for plan 'psi' (src/mod_fft.f90:482-487):
-  ! forward: kpsi = FFT(psi); inverse: psi = IFFT(kpsi)/(Nx*Ny)
+  ! fft(plan_name, direction) (src/mod_fft.f90:471-542).  Plan 'psi' acts
+  ! on the solver state (forward: kpsi = FFT(psi); inverse: psi =
+  ! IFFT(kpsi)/(Nx*Ny)) and refreshes the host copies.  The work-array plans
+  ! 'gpnlin', 'cmpxtmp', 'gptmp1'..'gptmp3' and 'gptransvobj' transform their
+  ! shared buffer in place on the GPU: forward takes the physical array to
+  ! the transposed Fourier array, unnormalised; inverse scales by 1/(Nx*Ny).
   SUBROUTINE fft(plan_name, direction)
     CHARACTER(*), INTENT(IN) :: plan_name
     INTEGER, INTENT(IN) :: direction
-    IF (TRIM(plan_name) /= 'psi') THEN
-      WRITE(*,*) 'fft: only plan psi is device resident in this layer'
+    INTEGER :: k
+    IF (TRIM(plan_name) == 'psi') THEN
+      IF (direction == forward) THEN
+        CALL vk_check(vk_gpe_set_psi(vkh, psi, 0_C_INT), 'fft forward')
+      ELSE
+        CALL vk_check(vk_gpe_set_kpsi(vkh, kpsi), 'fft inverse')
+      END IF
+      CALL sync_host()
+      RETURN
+    END IF
+    SELECT CASE (TRIM(plan_name))
+    CASE ('gpnlin'); k = 1
+    CASE ('cmpxtmp'); k = 2
+    CASE ('gptmp1'); k = 3
+    CASE ('gptmp2'); k = 4
+    CASE ('gptmp3'); k = 5
+    CASE ('gptransvobj'); k = 6
+    CASE DEFAULT
+      WRITE(*,*) 'fft: unknown plan ', TRIM(plan_name)
       STOP 2
-    END IF
-    IF (direction == forward) THEN
-      CALL vk_check(vk_gpe_set_psi(vkh, psi, 0_C_INT), 'fft forward')
-    ELSE
-      CALL vk_check(vk_gpe_set_kpsi(vkh, kpsi), 'fft inverse')
-    END IF
-    CALL sync_host()
+    END SELECT
+    CALL vk_check(vk_fft2d(vkh, INT(SIGN(1, direction), C_INT), C_LOC(work_store(1, k)), &
+                           C_LOC(work_store(1, k))), 'fft '//TRIM(plan_name))
   END SUBROUTINE
 
   ! restart_from_argle_data + ic_otoc_gpe: kpsi = FFT(psi), dealias, psi = IFFT(kpsi)
@@ -527,6 +574,11 @@ CONTAINS
     IF (C_ASSOCIATED(vkh)) CALL vk_gpe_destroy(vkh)
     vkh = C_NULL_PTR
     IF (ALLOCATED(psi)) DEALLOCATE(psi, kpsi, spec_gp)
+    IF (ALLOCATED(work_store)) THEN
+      NULLIFY(cgp_phys_nlin, cgp_four_nlin, phys_cmpxtmp, four_cmpxtmp, cgp_phys_tmp1, cgp_four_tmp1, &
+              cgp_phys_tmp2, cgp_four_tmp2, cgp_phys_tmp3, cgp_four_tmp3, cgp_phys_trans_vobj, cgp_four_trans_vobj)
+      DEALLOCATE(work_store)
+    END IF
     IF (ALLOCATED(gp_pos_obj)) DEALLOCATE(gp_pos_obj, gp_vel_obj, gp_f0ext_obj, gp_frc_obj)
   END SUBROUTINE
 

@@ -56,8 +56,20 @@ def test_fortran_driver(tmp_path, oracle_mod):
         sts.record()
     s2, s1 = sts.compute()
     f2 = np.frombuffer(raw[o:o + nt * sts.nshells * 8], dtype=np.float64).reshape(nt, sts.nshells)
-    f1 = np.frombuffer(raw[o + nt * sts.nshells * 8:], dtype=np.float64).reshape(nt, N)
+    o += nt * sts.nshells * 8
+    f1 = np.frombuffer(raw[o:o + nt * N * 8], dtype=np.float64).reshape(nt, N)
+    o += nt * N * 8
     assert np.array_equal(f2, s2) and np.array_equal(f1, s1)
+    # fft() work-array plans (src/mod_fft.f90:489-540): forward to the
+    # transposed Fourier layout, unnormalised; inverse scaled by 1/N^2
+    fb = N * N * 16
+    cplx = lambda k: np.frombuffer(raw[o + k * fb:o + (k + 1) * fb], dtype=np.complex128).reshape(N, N)
+    x, fnl, rt, fcm = cplx(0), cplx(1), cplx(2), cplx(3)
+    assert o + 4 * fb == raw.size
+    ref = np.fft.fft2(x).T                                # [kx, ky] rows: ky fastest
+    assert rel_l2(fnl, ref) < 1e-13
+    assert rel_l2(rt, x) < 1e-14
+    assert rel_l2(fcm, np.fft.fft2(x * x).T) < 1e-13
     sts.close()
     g.close()
 
