@@ -292,11 +292,16 @@ void colg_launch(vk_gpe* h, int kind, int npan, Kern k, A... args)
     ev_end(h, kind);
 }
 
-template <class C, class... A>
-void p2_launch(vk_gpe* h, A... args)
+// the flags (last argument: 0, 1 or 3) select a compile-time instantiation
+template <class C>
+void p2_launch(vk_gpe* h, cplx* Hrho, cplx* HV, const cplx* KV, cplx* Krho, const cplx* tw, Geo g, int flags)
 {
     constexpr int G = p2_cols<C>();
-    colg_launch<C, G>(h, PK_P2, h->geo.Pp, vk::k_p2<C, G>, args...);
+    auto go = [&](auto k) { colg_launch<C, G>(h, PK_P2, h->geo.Pp, k, Hrho, HV, KV, Krho, tw, g, flags); };
+    if (flags == 0) go(vk::k_p2<C, G, 0>);
+    else if (flags == 1) go(vk::k_p2<C, G, 1>);
+    else if (flags == 3) go(vk::k_p2<C, G, 3>);
+    else go(vk::k_p2<C, G>);
 }
 
 // The persistent prefetching P1 (vk::k_p1xp) runs at 4096^2 when the

@@ -426,11 +426,13 @@ __global__ void __launch_bounds__(C::ROWT) k_p1xp(const cplx* __restrict__ Hpsi,
 // (P_G), inverse along y, in place.  flags & 1: also the particle
 // potential column HV = IFFT_y(KV); flags & 2: store the dealiased
 // rho-hat row into K-space Krho (for the particle force).
-template <class C, int G = C::TC>
+// FL >= 0: the flags as a compile-time constant (flags is then ignored).
+template <class C, int G = C::TC, int FL = -1>
 __device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
-                                                const cplx* __restrict__ tw, Geo g, int flags)
+                                                const cplx* __restrict__ tw, Geo g, int flags_)
 {
+    const int flags = FL >= 0 ? FL : flags_;
     VK_COL_SETUP_G(C, G, blk_, pb_, g.Pp)
     cplx v[S];
 #pragma unroll
@@ -459,12 +461,12 @@ __device__ __forceinline__ void p2_body(int blk_, int nblk_, cplx* __restrict__ 
     }
 }
 
-template <class C, int G = C::TC>
+template <class C, int G = C::TC, int FL = -1>
 __global__ void __launch_bounds__(G * C::T) VK_COCC k_p2(cplx* __restrict__ Hrho, cplx* __restrict__ HV,
                                                 const cplx* __restrict__ KV, cplx* __restrict__ Krho,
                                                 const cplx* __restrict__ tw, Geo g, int flags)
 {
-    p2_body<C, G>(blockIdx.x, gridDim.x, Hrho, HV, KV, Krho, tw, g, flags);
+    p2_body<C, G, FL>(blockIdx.x, gridDim.x, Hrho, HV, KV, Krho, tw, g, flags);
 }
 
 // deterministic workgroup sum of NV values per thread; result valid in
