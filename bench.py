@@ -192,7 +192,8 @@ def ensemble_bench(G, torch, steps, warmup):
         g1.close()
     step_bytes = sum(per_size[N] for N in sizes)
     ach = step_bytes / (ms * 1e-3) / 1e9
-    names = ["all sizes per launch", "one launch per size", "one launch per realization"]
+    names = ["all sizes per launch", "one launch per size", "one launch per realization",
+             "one persistent launch per step"]
     return {"workload": "384 realizations (128 x 128^2, 128 x 256^2, 128 x 512^2), 8-64 vortices, RK4 dt=1e-3",
             "ensemble_steps_per_s": round(1e3 / ms, 2), "ms_per_step": round(ms, 4),
             "realization_steps_per_s": round(len(sizes) * 1e3 / ms, 1),
@@ -204,7 +205,7 @@ def ensemble_bench(G, torch, steps, warmup):
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 4), "compulsory_bytes_per_step": step_bytes},
             "autotune_ms_per_step": {names[m]: {"graph": round(float(tune[m][0]), 4), "direct": round(float(tune[m][1]), 4)}
-                                     for m in range(3)}}
+                                     for m in range(4)}}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk4_4096_traffic.txt")

@@ -207,10 +207,15 @@ int vk_pattern_bandwidth(int reps, double* gbps, int n);
 /* Ensemble of independent realizations (no reference counterpart: the
  * reference runs one field per MPI job).  Members have N in {128, 256, 512,
  * 1024}, no particles; each pass of an RK4 step is one launch over all
- * members (mode 0), one per size (mode 1) or one per member (mode 2), and a
- * step is replayed as a captured HIP graph (use_graph).  vk_ens_autotune
+ * members (mode 0), one per size (mode 1) or one per member (mode 2), or
+ * the whole step is one persistent launch (mode 3: as many workgroups as
+ * the device holds at once take the blocks of each pass in turn, with a
+ * grid barrier between passes; a barrier that waits longer than one second
+ * makes vk_ens_rk4_steps fail with VK_ERR_STATE and the state invalid), and
+ * a step is replayed as a captured HIP graph (use_graph).  vk_ens_autotune
  * times every schedule on the current state, restores the state and keeps
- * the fastest (ms_out[6] = ms/step for modes 0..2 x {graph, direct}). */
+ * the fastest (ms_out[8] = ms/step for modes 0..3 x {graph, direct}; -1 for
+ * a mode 3 whose barriers timed out). */
 typedef struct vk_ens vk_ens;
 int vk_ens_create(vk_ens** e, int nreal, const vk_gpe_params* params);
 void vk_ens_destroy(vk_ens* e);

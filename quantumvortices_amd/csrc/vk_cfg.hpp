@@ -72,6 +72,26 @@ template <> struct CfgE<512, 1> : CfgT<512, 8, 4, 4, true> {};
 template <> struct CfgE<1024, 1> : CfgT<1024, 8, 2, 2, true> {};
 constexpr bool has_variant(int N, int V) { return V == 0 || (V == 1 && (N == 256 || N == 512 || N == 1024)); }
 
+// The same shapes for pass bodies that run in a loop over blocks of several
+// shapes (the persistent ensemble step): the thread index is read through an
+// empty volatile asm, so that no value derived from it is hoisted out of the
+// loop.  Hoisted, every shape's index arithmetic would stay live across the
+// whole kernel and spill.
+template <int N, int V> struct CfgL : CfgE<N, V> { static constexpr bool OPAQUE_TID = true; };
+
+template <class C, class = void> struct opaque_tid { static constexpr bool value = false; };
+template <class C> struct opaque_tid<C, decltype((void)C::OPAQUE_TID)> { static constexpr bool value = C::OPAQUE_TID; };
+
+template <class C> __device__ __forceinline__ int tid_of()
+{
+    int u = threadIdx.x;
+    if constexpr (opaque_tid<C>::value) {
+        asm volatile("" : "+v"(u));
+        __builtin_assume(u >= 0 && u < 1024);
+    }
+    return u;
+}
+
 // Runtime panel geometry.  Retained |kx| <= M = floor(N/3) (the 2/3 rule
 // for 3 not dividing N).  Positive panels p < Pp cover kx in [0, M];
 // negative panels p >= NP - Pn cover kx in [N-M, N-1].

@@ -1588,9 +1588,16 @@ struct vk_ens {
     std::map<int, std::array<int*, 6>> smaps;     // N -> {map0, off0, map1, off1, map2, off2}
     std::map<int, std::array<int, 3>> snblk;
     size_t lds[4] = {};                           // max dynamic LDS per pass
-    int mode = 0;       // 0: one launch per pass for all sizes; 1: one per size; 2: per realization
+    // 0: one launch per pass for all sizes; 1: one per size; 2: per realization;
+    // 3: one persistent launch per step (all passes, grid barriers between)
+    int mode = 0;
     int use_graph = 1;
-    hipGraphExec_t gexec[3] = {};
+    hipGraphExec_t gexec[4] = {};
+    unsigned* dbar = nullptr;   // the persistent step's barrier counters
+    int* derr = nullptr;        // set by a barrier wait that ran out of time
+    int pgrid = 0;              // persistent grid: resident workgroups, multiple of 8
+    size_t plds = 0;
+    long long plimit = 0;       // barrier wait limit in wall-clock ticks
     int synced = 0;
     std::map<int, int> var;                       // N -> shape variant
 };
@@ -1850,6 +1857,22 @@ int vk_ens_create(vk_ens** out, int nreal, const vk_gpe_params* params)
     set_lds(vk::k_ens_p3, e->lds[2]);
     set_lds(vk::k_ens_p4, e->lds[3]);
     for (int N : e->sizes) dispatch(N, [&](auto n) { preconfig_passes<CfgN<decltype(n)::value>>(); return 0; });
+    // the persistent step: as many workgroups as the device holds at once
+    {
+        e->plds = *std::max_element(e->lds, e->lds + 4);
+        set_lds(vk::k_ens_step, e->plds);
+        int dev = 0, per_cu = 0, ncu = 0, rate_khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vk::k_ens_step, 256, e->plds) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "ens_create: device attributes"));
+        e->pgrid = (per_cu * ncu) / 8 * 8;
+        e->plimit = (long long)std::max(rate_khz, 1000) * 1000;   // one second
+        if (hipMalloc(&e->dbar, sizeof(unsigned) * vk::kEnsStepBarriers) != hipSuccess ||
+            hipMalloc(&e->derr, sizeof(int)) != hipSuccess || hipMemset(e->derr, 0, sizeof(int)) != hipSuccess)
+            return cleanup(fail(VK_ERR_HIP, "ens_create: barrier memory"));
+    }
     // shapes an earlier autotuning picked for this size on this GPU model
     for (int N : e->sizes) {
         const int v = cached_shape(N);
@@ -1869,6 +1892,8 @@ void vk_ens_destroy(vk_ens* e)
     for (int k = 0; k < 3; ++k) { if (e->dmap[k]) hipFree(e->dmap[k]); if (e->doff[k]) hipFree(e->doff[k]); }
     for (auto& kv : e->smaps) for (auto p : kv.second) if (p) hipFree(p);
     if (e->dR) hipFree(e->dR);
+    if (e->dbar) hipFree(e->dbar);
+    if (e->derr) hipFree(e->derr);
     for (auto h : e->h) vk_gpe_destroy(h);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -1877,6 +1902,18 @@ void vk_ens_destroy(vk_ens* e)
 }  // extern "C"
 
 namespace {
+
+// After the stream has drained: fail if a persistent step's barrier wait ran
+// out of time (a workgroup was not resident; that step's results are
+// invalid).  The flag is cleared for later steps.
+int ens_check_barrier(vk_ens* e)
+{
+    int err = 0;
+    HIPCHK(hipMemcpy(&err, e->derr, sizeof(int), hipMemcpyDeviceToHost));
+    if (!err) return 0;
+    HIPCHK(hipMemset(e->derr, 0, sizeof(int)));
+    return fail(VK_ERR_STATE, "ensemble mode 3: a grid barrier timed out (workgroups not co-resident); state invalid");
+}
 
 // one RK4 step of every realization on the ensemble stream
 int ens_step_launch(vk_ens* e)
@@ -1890,6 +1927,32 @@ int ens_step_launch(vk_ens* e)
             h->stream = keep;
             if (rc) return rc;
         }
+        return 0;
+    }
+    if (e->mode == 3) {
+        if (e->pgrid < 8) return fail(VK_ERR_UNSUPPORTED, "ensemble: the device holds too few workgroups for mode 3");
+        vk::EnsStepArgs A{};
+        // one group per member size, blocks numbered over the sizes ascending
+        if ((int)e->sizes.size() > vk::kEnsMaxGroups) return fail(VK_ERR_UNSUPPORTED, "ensemble: too many sizes");
+        int base[3] = {};
+        for (int N : e->sizes) {
+            vk::EnsGroup& G = A.grp[A.ngrp++];
+            G.key = N * 2 + (e->var.count(N) ? e->var[N] : 0);
+            const auto& m = e->smaps[N];
+            const auto& nb = e->snblk[N];
+            for (int k = 0; k < 3; ++k) {
+                G.map[k] = m[2 * k];
+                G.off[k] = m[2 * k + 1];
+                G.nblk[k] = nb[k];
+                G.base[k] = base[k];
+                base[k] += nb[k];
+            }
+        }
+        A.bar = e->dbar;
+        A.err = e->derr;
+        A.limit = e->plimit;
+        hipLaunchKernelGGL(vk::k_zero_u32, dim3(1), dim3(64), 0, e->stream, e->dbar, vk::kEnsStepBarriers);
+        hipLaunchKernelGGL(vk::k_ens_step, dim3(e->pgrid), dim3(256), e->plds, e->stream, (const vk::EnsReal*)e->dR, A);
         return 0;
     }
     struct Grp { int* m[3]; int* o[3]; int nb[3]; };
@@ -2056,12 +2119,12 @@ int vk_ens_rk4_steps(vk_ens* e, int nsteps)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipGetLastError());
-    return 0;
+    return ens_check_barrier(e);
 }
 
 int vk_ens_set_mode(vk_ens* e, int mode, int use_graph)
 {
-    if (!e || mode < 0 || mode > 2) return fail(VK_ERR_ARG, "ens_set_mode: bad arguments");
+    if (!e || mode < 0 || mode > 3) return fail(VK_ERR_ARG, "ens_set_mode: bad arguments");
     HIPCHK(hipStreamSynchronize(e->stream));
     if (mode != e->mode) e->synced = 0;   // the per-member mode uses the single-field layout
     e->mode = mode;
@@ -2078,8 +2141,8 @@ int vk_ens_get_mode(vk_ens* e, int* mode, int* use_graph)
 }
 
 // Autotuner: times every (launch mode, graph) schedule over nsteps steps from
-// the current state, restores the state, keeps the fastest.  ms_out[6]:
-// ms per step for modes 0,1,2 x {graph, direct}.
+// the current state, restores the state, keeps the fastest.  ms_out[8]:
+// ms per step for modes 0..3 x {graph, direct}.
 int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
 {
     if (!e || nsteps <= 0) return fail(VK_ERR_ARG, "ens_autotune: bad arguments");
@@ -2099,7 +2162,7 @@ int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
     HIPCHK(hipEventCreate(&b));
     double best = 1e300;
     int bm = 0, bg = 1;
-    for (int m = 0; m < 3; ++m)
+    for (int m = 0; m < 4; ++m)
         for (int gph = 1; gph >= 0; --gph) {
             if (m != e->mode) e->synced = 0;
             e->mode = m;
@@ -2109,6 +2172,10 @@ int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
             if ((rc = ens_steps_async(e, nsteps))) return rc;
             HIPCHK(hipEventRecord(b, e->stream));
             HIPCHK(hipEventSynchronize(b));
+            if (ens_check_barrier(e)) {   // mode 3 not co-resident here: not a candidate
+                if (ms_out) ms_out[2 * m + (1 - gph)] = -1.0;
+                continue;
+            }
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, a, b));
             const double per = ms / nsteps;

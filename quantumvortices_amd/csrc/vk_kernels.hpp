@@ -70,7 +70,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     constexpr int NP = C::NP;                                              \
     (void)NP;                                                              \
     extern __shared__ double lds[];                                        \
-    const int u = threadIdx.x;                                             \
+    const int u = tid_of<C>();                                             \
     const int q_ = u % (TR * TC);                                          \
     const int rr = q_ / TC, c = q_ % TC, pidx0 = u / (TR * TC);            \
     const int t = TC * pidx0 + c;                                          \
@@ -82,7 +82,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     constexpr int NP = C::NP;                                              \
     (void)NP;                                                              \
     extern __shared__ double lds[];                                        \
-    const int u = threadIdx.x;                                             \
+    const int u = tid_of<C>();                                             \
     const int c = u % TC, t = u / TC;                                      \
     const int p = (PEXPR);                                                 \
     const int kx = p * TC + c;                                             \
@@ -107,7 +107,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
         if (pb_ >= (NPAN)) return;                                         \
     }                                                                      \
     const int p = (PEXPR);                                                 \
-    const int u = threadIdx.x;                                             \
+    const int u = tid_of<C>();                                             \
     const int cl_ = u % (G), t = u / (G);                                  \
     const int c = hh_ * (G) + cl_;                                         \
     const int kx = p * TC + c;                                             \

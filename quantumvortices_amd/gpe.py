@@ -511,9 +511,9 @@ class Ensemble:
         return m.value, bool(g.value)
 
     def autotune(self, nsteps=3):
-        ms = np.zeros(6)
+        ms = np.zeros(8)
         nat.check(nat.lib().vk_ens_autotune(self.h, int(nsteps), nat.ptr(ms)))
-        return ms.reshape(3, 2)
+        return ms.reshape(4, 2)
 
     def set_shape(self, N, var):
         nat.check(nat.lib().vk_ens_set_shape(self.h, int(N), int(var)))

@@ -1,6 +1,7 @@
 """Batched ensemble of unequal realizations: every schedule (one launch per
-pass for all sizes, per size, per realization; graph replay or direct) gives
-each member exactly its single-field result, and members match the oracle."""
+pass for all sizes, per size, per realization, or one persistent launch per
+step; graph replay or direct) gives each member exactly its single-field
+result, and members match the oracle."""
 import sys
 
 import numpy as np
@@ -44,7 +45,7 @@ def G():
     return gpe
 
 
-@pytest.mark.parametrize("mode,graph", [(0, True), (0, False), (1, True), (2, True), (2, False)])
+@pytest.mark.parametrize("mode,graph", [(0, True), (0, False), (1, True), (2, True), (2, False), (3, True), (3, False)])
 def test_ensemble_equals_single(G, mode, graph):
     say("case ensemble mode=%d graph=%s" % (mode, graph))
     sizes, ps, ics = members(G)
@@ -144,8 +145,9 @@ def test_ensemble_shapes(G, oracle_mod):
     """Members of sizes 256..1024 in their second shape (8 points per
     thread, radix-8 stages, half-width panels): within roundoff of the
     single-field runs and within 1e-11 of the oracle; members left in
-    shape 0 stay bit-identical; reading a member between steps and the
-    per-member launch mode keep the state consistent; the shape autotuner
+    shape 0 stay bit-identical; reading a member between steps, the
+    per-member launch mode and the persistent step keep the state
+    consistent; the shape autotuner
     restores the state and reports both timings."""
     say("case ensemble shapes")
     sizes, ps, ics = members(G)
@@ -161,8 +163,10 @@ def test_ensemble_shapes(G, oracle_mod):
     _ = e.psi(3)              # a shape-1 member read between steps
     e.set_mode(2, False)
     e.rk4(1)                  # per-member launches use the single-field layout
+    e.set_mode(3, True)
+    e.rk4(1)                  # the persistent step over shape-1 members
     e.set_mode(0, True)
-    e.rk4(2)
+    e.rk4(1)
     for r, N in enumerate(sizes):
         a = e.psi(r)
         if N == 128:
