@@ -213,11 +213,32 @@ PASS_KERNEL = {"p1_rho_row": "vk::k_p1x<4096, TR=1 >", "p2_dealias_col": "vk::k_
                "p3_nonlin_row": "vk::k_p3x<4096, 6>", "p4_rk4_col": "vk::k_p4<4096, avg>"}
 
 
-def measured_traffic(pass_name, N):
+# row-pass variants the traffic summary was taken with (P1 one row per
+# workgroup, P3 two rows)
+PROFILED_ROWS = (1, 0)
+ROW_P1_NAMES = ["P1 TR rows per workgroup", "P1 one row per workgroup", "P1 persistent with LDS prefetch"]
+ROW_P3_NAMES = ["P3 TR rows per workgroup", "P3 one row per workgroup"]
+
+
+def tune_rows(g):
+    """Row-pass variant autotuning (N >= 2048): ((p1, p3), ms table), or
+    (None, None) where the size has one variant."""
+    from quantumvortices_amd import _native as nat
+    try:
+        return g.autotune_rows(3)
+    except nat.VKError:
+        return None, None
+
+
+def measured_traffic(pass_name, N, rows=PROFILED_ROWS):
     """HBM bytes per launch of a pass at 4096^2 with particles, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
-    device copy); None for other sizes or when the summary is missing."""
+    device copy); None for other sizes, for a row-pass variant other than
+    the profiled one, or when the summary is missing."""
     if N != 4096 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    if (pass_name == "p1_rho_row" and rows[0] != PROFILED_ROWS[0]) or \
+            (pass_name == "p3_nonlin_row" and rows[1] != PROFILED_ROWS[1]):
         return None
     for line in open(TRAFFIC_FILE):
         if line.startswith(PASS_KERNEL.get(pass_name, "?") + " "):
@@ -390,6 +411,9 @@ def main():
 
     log("bench: building %d^2 workload with particles" % N)
     g, wl = make_workload(G, N, dt, particles=True)
+    # row-pass variants: timed from the initial state, which is restored
+    rows, rows_ms = tune_rows(g)
+    log("bench: row variants (p1, p3) =", rows, None if rows_ms is None else rows_ms.tolist())
     ms = time_steps(torch, g, args.steps, args.warmup)
     steps_per_s = args.steps / (ms / 1e3)
     log("bench: particles %d steps in %.1f ms -> %.2f steps/s" % (args.steps, ms, steps_per_s))
@@ -402,8 +426,10 @@ def main():
 
     pure = None
     pure_pt = None
+    pure_rows = None
     if not args.no_pure:
         gp, _ = make_workload(G, N, dt, particles=False)
+        pure_rows, _ = tune_rows(gp)
         pms = time_steps(torch, gp, args.steps, args.warmup)
         pure = args.steps / (pms / 1e3)
         pure_pt = gp.pass_times(max(2, min(10, args.steps)))
@@ -415,7 +441,7 @@ def main():
     dom = max(passes, key=lambda k: pt[k])
     achieved = pb[dom] / (pt[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, N),
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, N, rows or PROFILED_ROWS),
             "bytes_per_launch": pb[dom], "avg_ms": pt[dom]}
     per_pass = {k: {"ms": round(pt[k], 4), "GB/s": round(pb[k] / (pt[k] * 1e-3) / 1e9, 1) if pt[k] > 0 else None}
                 for k in passes}
@@ -438,7 +464,7 @@ def main():
     # four stages (the particle kernels move a few MB), and the bytes the
     # counters measured for the same passes
     step_bytes = 4.0 * sum(pb[k] for k in passes)
-    meas = [measured_traffic(k, N) for k in passes]
+    meas = [measured_traffic(k, N, rows or PROFILED_ROWS) for k in passes]
     step_meas = 4.0 * sum(meas) if all(m is not None for m in meas) else None
     step_gbs = step_bytes * steps_per_s / 1e9
     step_roof = {"bound": "hbm", "compulsory_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
@@ -503,6 +529,12 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "per_pass": per_pass,
+        "row_variants": None if rows is None else {
+            "p1": ROW_P1_NAMES[rows[0]], "p3": ROW_P3_NAMES[rows[1]], "TR": {4096: 2, 2048: 4}.get(N),
+            "ms_per_step": {"%s / %s" % (ROW_P1_NAMES[a], ROW_P3_NAMES[b]): round(float(rows_ms[a][b]), 4)
+                            for a in range(3) for b in range(2) if rows_ms[a][b] > 0},
+            "pure_gpe": None if (pure is None or pure_rows is None) else
+            {"p1": ROW_P1_NAMES[pure_rows[0]], "p3": ROW_P3_NAMES[pure_rows[1]]}},
         "pure_gpe_steps_per_s": round(pure, 3) if pure else None,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,

@@ -193,6 +193,20 @@ int vk_gpe_set_ic_vortices(vk_gpe* h, int nv, const double* xs,
  * vk_pass_name); the state advances by nsteps. */
 void* vk_gpe_stream(vk_gpe* h);
 int vk_gpe_pass_times(vk_gpe* h, int nsteps, double* ms_out, int nkinds);
+/* Row-pass variants of the psi-row path (N >= 2048; no reference
+ * counterpart, the reference has one FFT plan per size).  p1: 0 TR rows per
+ * workgroup, 1 one row per workgroup, 2 the persistent P1 that prefetches
+ * its next rows into LDS (4096^2 on a device whose CU count divides the row
+ * blocks evenly over the XCDs); p3: 0 TR rows, 1 one row.  The variants
+ * order floating-point sums differently: results agree to roundoff, not
+ * bitwise.  vk_gpe_autotune_rows times nsteps RK4 steps with every variant
+ * pair from the current state, restores the state (kpsi, its y-inverse and
+ * the particle state), keeps the fastest pair (choice[2] = {p1, p3}) and
+ * returns ms/step in ms_out[2*p1 + p3] (6 entries, -1 where a variant does
+ * not exist). */
+int vk_gpe_set_row_variants(vk_gpe* h, int p1, int p3);
+int vk_gpe_get_row_variants(vk_gpe* h, int* p1, int* p3);
+int vk_gpe_autotune_rows(vk_gpe* h, int nsteps, int* choice, double* ms_out);
 const char* vk_pass_name(int kind);
 int vk_pass_kinds(void);
 /* Compulsory HBM bytes one launch of kernel kind moves at this handle's N. */

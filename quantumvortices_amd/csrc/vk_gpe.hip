@@ -310,18 +310,24 @@ void p2_launch(vk_gpe* h, cplx* Hrho, cplx* HV, const cplx* KV, cplx* Krho, cons
 #ifndef VK_P1_PREFETCH
 #define VK_P1_PREFETCH 1
 #endif
-int p1_prefetch_ok(const vk_gpe* h)
+// the persistent prefetching P1 fits this handle and device (4096^2, a CU
+// count whose row blocks divide evenly over the XCDs)
+int p1_prefetch_fits(const vk_gpe* h)
 {
     if (!VK_P1_PREFETCH || h->N != 4096 || !h->xpsi) return 0;
-    // the plain P1, whose loads are all in flight at once, is faster; the
-    // prefetching kernel runs when VK_P1_PREFETCH=1 (tests compare the two)
-    const char* on = std::getenv("VK_P1_PREFETCH");
-    if (!on || on[0] != '1') return 0;
     using C = CfgN<4096>;
     static_assert(vk::p1p_lds<C>() <= 160 * 1024, "prefetch buffer exceeds LDS");
     const int g8 = h->ncu / 8, r8 = (C::N / C::TR) / 8;
     return h->ncu > 0 && h->ncu % 8 == 0 && r8 % g8 == 0 && h->geo.Pp == vk::geo_pp<C>() &&
            h->geo.Pn == vk::geo_pn<C>();
+}
+
+int p1_prefetch_ok(const vk_gpe* h)
+{
+    // the plain P1, whose loads are all in flight at once, is faster; the
+    // prefetching kernel runs when VK_P1_PREFETCH=1 (tests compare the two)
+    const char* on = std::getenv("VK_P1_PREFETCH");
+    return on && on[0] == '1' && p1_prefetch_fits(h);
 }
 
 template <class C>
@@ -922,11 +928,12 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         h->p1pf = p1_prefetch_ok(h);
         const char* t1 = std::getenv("VK_P1_TR1");
         const char* t3 = std::getenv("VK_P3_TR1");
-        // P1 with one row per workgroup (three 256-thread workgroups per CU)
-        // is 6% faster at 4096^2 and the default there; P3 is faster with two
-        // rows.  VK_P1_TR1 / VK_P3_TR1 = 0 or 1 override.
-        h->tr1_p1 = t1 ? t1[0] == '1' : N == 4096;
-        h->tr1_p3 = t3 && t3[0] == '1';
+        // Measured defaults: one row per workgroup in P1 and P3 at 2048^2
+        // (13% faster per step than four rows); at 4096^2 one row in P1 (6%
+        // faster) and two in P3.  VK_P1_TR1 / VK_P3_TR1 = 0 or 1 override;
+        // vk_gpe_autotune_rows times every pair on the device at hand.
+        h->tr1_p1 = t1 ? t1[0] == '1' : (N == 4096 || N == 2048);
+        h->tr1_p3 = t3 ? t3[0] == '1' : N == 2048;
     }
     // psi rows: the single-field RK4/ARGLE path at N >= VK_XPSI_MIN_N and the
     // ensemble kernels (which also step single fields of N <= 1024)
@@ -1539,6 +1546,98 @@ int vk_gpe_pass_times(vk_gpe* h, int nsteps, double* ms_out, int nkinds)
         cnt[h->evkind[i]]++;
     }
     for (int k = 0; k < nkinds && k < PK_COUNT; ++k) ms_out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    return 0;
+}
+
+int vk_gpe_set_row_variants(vk_gpe* h, int p1, int p3)
+{
+    if (!h || p1 < 0 || p1 > 2 || p3 < 0 || p3 > 1) return fail(VK_ERR_ARG, "set_row_variants: bad arguments");
+    if (!h->xpsi) return fail(VK_ERR_UNSUPPORTED, "set_row_variants: the row-pass variants exist for the psi-row path only");
+    if (p1 == 2 && !p1_prefetch_fits(h))
+        return fail(VK_ERR_UNSUPPORTED, "set_row_variants: the prefetching P1 does not fit this size or device");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->p1pf = p1 == 2;
+    h->tr1_p1 = p1 == 1;
+    h->tr1_p3 = p3;
+    return 0;
+}
+
+int vk_gpe_get_row_variants(vk_gpe* h, int* p1, int* p3)
+{
+    if (!h) return fail(VK_ERR_ARG, "get_row_variants: null");
+    if (p1) *p1 = h->p1pf ? 2 : (h->tr1_p1 ? 1 : 0);
+    if (p3) *p3 = h->tr1_p3;
+    return 0;
+}
+
+int vk_gpe_autotune_rows(vk_gpe* h, int nsteps, int* choice, double* ms_out)
+{
+    if (!h || nsteps <= 0) return fail(VK_ERR_ARG, "autotune_rows: bad arguments");
+    if (!h->xpsi) return fail(VK_ERR_UNSUPPORTED, "autotune_rows: the row-pass variants exist for the psi-row path only");
+    int rc = ensure_rk4(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // save the state: kpsi, its y-inverse, the particle state
+    const size_t nf = f2(h), npst = h->pst ? (size_t)30 * h->np_cap + 8 : 0;
+    cplx* sv = nullptr;
+    double* sp = nullptr;
+    HIPCHK(hipMalloc(&sv, 2 * nf * sizeof(cplx)));
+    if (npst && hipMalloc(&sp, npst * sizeof(double)) != hipSuccess) {
+        hipFree(sv);
+        return fail(VK_ERR_HIP, "autotune_rows: hipMalloc");
+    }
+    auto release = [&]() { hipFree(sv); if (sp) hipFree(sp); };
+    auto copy = [&](bool save) -> int {
+        HIPCHK(hipMemcpyAsync(save ? sv : h->K0, save ? h->K0 : sv, nf * sizeof(cplx), hipMemcpyDeviceToDevice,
+                              h->stream));
+        HIPCHK(hipMemcpyAsync(save ? sv + nf : h->Hpsi, save ? h->Hpsi : sv + nf, nf * sizeof(cplx),
+                              hipMemcpyDeviceToDevice, h->stream));
+        if (npst)
+            HIPCHK(hipMemcpyAsync(save ? sp : h->pst, save ? h->pst : sp, npst * sizeof(double),
+                                  hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    };
+    if ((rc = copy(true))) { release(); return rc; }
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        release();
+        return fail(VK_ERR_HIP, "autotune_rows: events");
+    }
+    const int keep1 = h->p1pf ? 2 : (h->tr1_p1 ? 1 : 0), keep3 = h->tr1_p3;
+    double best = 1e300;
+    int b1 = keep1, b3 = keep3;
+    for (int p1 = 0; p1 <= 2 && !rc; ++p1)
+        for (int p3 = 0; p3 <= 1 && !rc; ++p3) {
+            if (ms_out) ms_out[2 * p1 + p3] = -1.0;
+            if (p1 == 2 && !p1_prefetch_fits(h)) continue;
+            h->p1pf = p1 == 2;
+            h->tr1_p1 = p1 == 1;
+            h->tr1_p3 = p3;
+            if ((rc = vk_gpe_rk4_steps_async(h, 1))) break;   // warm
+            if (hipEventRecord(a, h->stream) != hipSuccess) { rc = fail(VK_ERR_HIP, "autotune_rows: event"); break; }
+            if ((rc = vk_gpe_rk4_steps_async(h, nsteps))) break;
+            if (hipEventRecord(b, h->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+                rc = fail(VK_ERR_HIP, "autotune_rows: event");
+                break;
+            }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, a, b);
+            const double per = ms / nsteps;
+            if (ms_out) ms_out[2 * p1 + p3] = per;
+            if (per < best) { best = per; b1 = p1; b3 = p3; }
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    h->p1pf = b1 == 2;
+    h->tr1_p1 = b1 == 1;
+    h->tr1_p3 = b3;
+    if (!rc) rc = copy(false);
+    release();
+    if (rc) return rc;
+    h->hstate = 2;
+    h->kv_valid = 0;
+    if (choice) { choice[0] = b1; choice[1] = b3; }
     return 0;
 }
 

@@ -285,6 +285,26 @@ class GPE:
         names = [nat.lib().vk_pass_name(i).decode() for i in range(k)]
         return dict(zip(names, ms.tolist()))
 
+    def set_row_variants(self, p1, p3):
+        """Row-pass variants of the psi-row path (N >= 2048): p1 0 = TR rows
+        per workgroup, 1 = one row, 2 = the prefetching persistent P1;
+        p3 0 = TR rows, 1 = one row."""
+        nat.check(nat.lib().vk_gpe_set_row_variants(self.h, int(p1), int(p3)))
+
+    def row_variants(self):
+        a, b = nat.ctypes.c_int(), nat.ctypes.c_int()
+        nat.check(nat.lib().vk_gpe_get_row_variants(self.h, nat.ctypes.byref(a), nat.ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune_rows(self, nsteps=3):
+        """Time every row-pass variant pair from the current state, restore
+        the state and keep the fastest: ((p1, p3), ms/step [p1][p3], -1
+        where a variant does not exist)."""
+        ch = (nat.ctypes.c_int * 2)()
+        ms = np.zeros(6)
+        nat.check(nat.lib().vk_gpe_autotune_rows(self.h, int(nsteps), ch, nat.ptr(ms)))
+        return (ch[0], ch[1]), ms.reshape(3, 2)
+
     def pass_bytes(self):
         k = nat.lib().vk_pass_kinds()
         names = [nat.lib().vk_pass_name(i).decode() for i in range(k)]
