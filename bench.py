@@ -180,6 +180,17 @@ def ensemble_bench(G, torch, steps, warmup):
     ms = e0.elapsed_time(e1) / steps
     mode, graph = e.mode()
     points = sum(N * N for N in sizes)
+    # every member's diagnostics (energies and six spectra): one call for
+    # all members, then one call per member (the first call allocates the
+    # diagnostics' work fields and is not timed)
+    e.diagnostics_all()
+    t0 = time.perf_counter()
+    e.diagnostics_all()
+    diag_all_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    for r in range(len(sizes)):
+        e.diagnostics(r)
+    diag_loop_ms = (time.perf_counter() - t0) * 1e3
     e.close()
     # compulsory HBM bytes of one step of every member: the four passes of
     # the four stages, per member size (the members run the same pass
@@ -199,6 +210,8 @@ def ensemble_bench(G, torch, steps, warmup):
             "realization_steps_per_s": round(len(sizes) * 1e3 / ms, 1),
             "grid_point_steps_per_s": float("%.4g" % (points * 1e3 / ms)),
             "schedule": {"launch": names[mode], "graph": graph},
+            "diagnostics_all_members_ms": {"one call": round(diag_all_ms, 2),
+                                           "one call per member": round(diag_loop_ms, 2)},
             "shapes": {str(N): {"chosen": ("8 points per thread" if v[0] else "16 points per thread"),
                                 "ms_16pt": round(v[1], 4) if v[1] > 0 else None,
                                 "ms_8pt": round(v[2], 4) if v[2] > 0 else None} for N, v in shapes.items()},

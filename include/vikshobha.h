@@ -240,6 +240,12 @@ int vk_ens_set_ic_vortices(vk_ens* e, int r, int nv, const double* xs, const dou
                            const int* qs, int nimg, int flags);
 int vk_ens_get_psi(vk_ens* e, int r, double* psi_xy, double* kpsi_t);
 int vk_ens_diagnostics(vk_ens* e, int r, double* scal, double* spec, double* pscal);
+/* vk_ens_diagnostics of every member at once: the members' pipelines run
+ * concurrently on their own streams with one wait at the end.  scal[8 r ..]
+ * holds member r's scalars; member r's six spectra start at
+ * spec[sum over q < r of 6 (nshell(N_q) + 1)] (the layout vk_gpe_diagnostics
+ * gives one member). */
+int vk_ens_diagnostics_all(vk_ens* e, double* scal, double* spec);
 int vk_ens_rk4_steps(vk_ens* e, int nsteps);
 int vk_ens_rk4_steps_async(vk_ens* e, int nsteps);
 int vk_ens_set_mode(vk_ens* e, int mode, int use_graph);

@@ -85,6 +85,7 @@ SIGNATURES = {
     "vk_ens_set_ic_vortices": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, dp, ip, ctypes.c_int, ctypes.c_int]),
     "vk_ens_get_psi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp]),
     "vk_ens_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]),
+    "vk_ens_diagnostics_all": (ctypes.c_int, [ctypes.c_void_p, dp, dp]),
     "vk_ens_rk4_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "vk_ens_rk4_steps_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "vk_ens_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),

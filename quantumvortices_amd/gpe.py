@@ -512,6 +512,19 @@ class Ensemble:
         nat.check(nat.lib().vk_ens_diagnostics(self.h, int(r), nat.ptr(scal), nat.ptr(spec), nat.ptr(pscal)))
         return scal, spec.reshape(6, ns + 1)
 
+    def diagnostics_all(self):
+        """diagnostics(r) of every member in one call: a list of
+        (scalars[8], spectra[6, nshell+1]) per member."""
+        ns = [nshell(p.N) for p in self.params]
+        scal = np.zeros(8 * len(ns))
+        spec = np.zeros(sum(6 * (n + 1) for n in ns))
+        nat.check(nat.lib().vk_ens_diagnostics_all(self.h, nat.ptr(scal), nat.ptr(spec)))
+        out, o = [], 0
+        for r, n in enumerate(ns):
+            out.append((scal[8 * r:8 * r + 8], spec[o:o + 6 * (n + 1)].reshape(6, n + 1)))
+            o += 6 * (n + 1)
+        return out
+
     def rk4(self, nsteps):
         nat.check(nat.lib().vk_ens_rk4_steps(self.h, int(nsteps)))
 
