@@ -1716,7 +1716,10 @@ struct vk_ens {
     size_t plds = 0;
     long long plimit = 0;       // barrier wait limit in wall-clock ticks
     double* hstage = nullptr;   // pinned staging of every member's diagnostics
+    double* dstage = nullptr;   // the same on the device
     size_t nstage = 0;
+    void* dtab = nullptr;       // operand tables of the batched diagnostics
+    size_t ntab = 0;
     int synced = 0;
     std::map<int, int> var;                       // N -> shape variant
 };
@@ -1941,6 +1944,86 @@ int col_inv_var(vk_gpe* h, int var)
     });
 }
 
+// Diagnostics of the members hs (all of size C::N, all with Hpsi in layout
+// pl: 1 retained panels, 0 all panels) as batched launches on stream st:
+// the single-field pipeline (diag_launch) with blockIdx.y selecting the
+// member, so each member gets bit for bit its single-field result.  Member
+// i's sums go to sums[i] (8 doubles) and its spectra to spec[i].
+// tab: device memory for the operand tables, ens_diag_tab_bytes(hs.size()).
+constexpr int kDiagPairSteps = 16;
+size_t ens_diag_tab_bytes(size_t n)
+{
+    return ((kDiagPairSteps * n * sizeof(vk::BPair) + 255) / 256) * 256 + n * sizeof(vk::DiagB);
+}
+
+template <class C>
+int ens_diag_group(const std::vector<vk_gpe*>& hs, int pl, hipStream_t st, void* tab,
+                   const std::vector<double*>& sums, const std::vector<double*>& spec)
+{
+    const int n = (int)hs.size();
+    const vk_gpe* h0 = hs[0];
+    const Geo g = h0->geo;
+    const cplx* tw = h0->tw;   // the same table in every member of this size
+    const double sc = scale_of(h0);
+    std::vector<vk::BPair> pairs((size_t)kDiagPairSteps * n);
+    std::vector<vk::DiagB> db(n);
+    for (int i = 0; i < n; ++i) {
+        vk_gpe* h = hs[i];
+        cplx** X = h->X;
+        cplx** Kd = h->Kd;
+        const vk::BPair row[kDiagPairSteps] = {
+            {h->Hpsi, X[0]}, {h->Hpsi, X[1]}, {h->K0, h->Hd}, {h->Hd, X[2]},   // psi, d_x psi, d_y psi
+            {X[0], h->Hd}, {h->Hd, Kd[4]},                                     // rho-hat, dealiased
+            {Kd[4], h->Hd}, {h->Hd, X[3]},                                     // P_G rho
+            {X[1], h->Hd}, {h->Hd, Kd[0]}, {X[2], h->Hd}, {h->Hd, Kd[1]},       // w, q transforms
+            {X[5], h->Hd}, {h->Hd, Kd[2]}, {X[6], h->Hd}, {h->Hd, Kd[3]}};
+        for (int k = 0; k < kDiagPairSteps; ++k) pairs[(size_t)k * n + i] = row[k];
+        vk::DiagB& d = db[i];
+        d.psi = X[0]; d.D1 = X[1]; d.D2 = X[2]; d.PR = X[3];
+        d.W1 = X[1]; d.W2 = X[2]; d.Q1 = X[5]; d.Q2 = X[6];
+        d.part = h->dpart;
+        d.sums = sums[i];
+        d.spec = spec[i];
+        d.KW1 = Kd[0]; d.KW2 = Kd[1]; d.KQ1 = Kd[2]; d.KQ2 = Kd[3]; d.KR = Kd[4]; d.KP = h->K0;
+        d.da = vk::DiagArgs{h->p.alpha, h->p.omega, h->p.g, (double)1e-30f, h->N, 0};
+        const double en = 1.0 / ((double)h->N * (double)h->N);
+        d.sa = vk::SpecArgs{h->p.alpha, h->p.g, en * en, h->N, h->nshell};
+    }
+    char* base = static_cast<char*>(tab);
+    vk::BPair* dp = reinterpret_cast<vk::BPair*>(base);
+    vk::DiagB* dd = reinterpret_cast<vk::DiagB*>(base + ((kDiagPairSteps * n * sizeof(vk::BPair) + 255) / 256) * 256);
+    // pageable sources: the copies complete before the calls return
+    HIPCHK(hipMemcpyAsync(dp, pairs.data(), pairs.size() * sizeof(vk::BPair), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dd, db.data(), db.size() * sizeof(vk::DiagB), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const size_t rl = (size_t)C::TR * C::ROWRS * sizeof(double), cl = (size_t)C::TC * C::COLRS * sizeof(double);
+    set_lds(vk::k_row_inv_b<C>, rl);
+    set_lds(vk::k_row_fwd_b<C>, rl);
+    set_lds(vk::k_col_inv_b<C>, cl);
+    set_lds(vk::k_col_fwd_b<C>, cl);
+    const dim3 rg(C::N / C::TR, n), rb(C::ROWT), cb(C::COLT);
+    const int ncl = pl == 1 ? g.Pp + g.Pn : C::NP;
+    auto P = [&](int k) { return (const vk::BPair*)(dp + (size_t)k * n); };
+    hipLaunchKernelGGL(vk::k_row_inv_b<C>, rg, rb, rl, st, P(0), tw, g, pl, 0, sc);
+    hipLaunchKernelGGL(vk::k_row_inv_b<C>, rg, rb, rl, st, P(1), tw, g, pl, 1, sc);
+    hipLaunchKernelGGL(vk::k_col_inv_b<C>, dim3(ncl, n), cb, cl, st, P(2), tw, g, pl, 2);
+    hipLaunchKernelGGL(vk::k_row_inv_b<C>, rg, rb, rl, st, P(3), tw, g, pl, 0, sc);
+    hipLaunchKernelGGL(vk::k_row_fwd_b<C>, rg, rb, rl, st, P(4), tw, g, 2, 1);
+    hipLaunchKernelGGL(vk::k_col_fwd_b<C>, dim3(g.Pp, n), cb, cl, st, P(5), tw, g, 2, 1);
+    hipLaunchKernelGGL(vk::k_col_inv_b<C>, dim3(g.Pp, n), cb, cl, st, P(6), tw, g, 2, 0);
+    hipLaunchKernelGGL(vk::k_row_inv_b<C>, rg, rb, rl, st, P(7), tw, g, 2, 0, sc);
+    const int nb = 1024;   // as the single-field pass: the same partial sums
+    hipLaunchKernelGGL(vk::k_diag_x_b, dim3(nb, n), dim3(256), 0, st, (const vk::DiagB*)dd);
+    hipLaunchKernelGGL(vk::k_reduce_fixed_b, dim3(n), dim3(256), 0, st, (const vk::DiagB*)dd, nb);
+    for (int k = 8; k < kDiagPairSteps; k += 2) {
+        hipLaunchKernelGGL(vk::k_row_fwd_b<C>, rg, rb, rl, st, P(k), tw, g, 2, 0);
+        hipLaunchKernelGGL(vk::k_col_fwd_b<C>, dim3(g.Pp, n), cb, cl, st, P(k + 1), tw, g, 2, 0);
+    }
+    hipLaunchKernelGGL(vk::k_spectra_b, dim3(h0->nshell + 1, n), dim3(256), 0, st, (const int*)h0->modes,
+                       (const int*)h0->offs, (const vk::DiagB*)dd);
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2014,6 +2097,8 @@ void vk_ens_destroy(vk_ens* e)
     if (e->dbar) hipFree(e->dbar);
     if (e->derr) hipFree(e->derr);
     if (e->hstage) hipHostFree(e->hstage);
+    if (e->dstage) hipFree(e->dstage);
+    if (e->dtab) hipFree(e->dtab);
     for (auto h : e->h) vk_gpe_destroy(h);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -2209,39 +2294,80 @@ int vk_ens_diagnostics_all(vk_ens* e, double* scal, double* spec)
 {
     if (!e) return fail(VK_ERR_ARG, "ens_diagnostics_all: null");
     HIPCHK(hipStreamSynchronize(e->stream));
-    size_t need = 0;
-    for (auto h : e->h) need += 8 + 6 * (size_t)(h->nshell + 1);
+    const size_t nreal = e->h.size();
+    std::vector<size_t> offs(nreal + 1, 0);
+    for (size_t r = 0; r < nreal; ++r) offs[r + 1] = offs[r] + 8 + 6 * (size_t)(e->h[r]->nshell + 1);
+    const size_t need = offs[nreal];
     if (need > e->nstage) {
         if (e->hstage) hipHostFree(e->hstage);
+        if (e->dstage) hipFree(e->dstage);
         e->hstage = nullptr;
+        e->dstage = nullptr;
         e->nstage = 0;
         HIPCHK(hipHostMalloc((void**)&e->hstage, need * sizeof(double)));
+        HIPCHK(hipMalloc(&e->dstage, need * sizeof(double)));
         e->nstage = need;
     }
-    // every member's pipeline on its own stream, results staged in pinned
-    // memory; one wait at the end (issuing the launches from several host
-    // threads was slower: 88 against 54 ms for 384 members)
-    size_t off = 0;
     for (auto h : e->h) {
-        const size_t ns = 6 * (size_t)(h->nshell + 1);
-        int rc = diag_launch(h);
+        int rc = diag_setup(h);
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(e->hstage + off, h->dsum, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(e->hstage + off + 8, h->dspec, ns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        off += 8 + ns;
-        if (h->ens_var) e->synced = 0;   // Hpsi rebuilt in the single-field layout
     }
-    for (auto h : e->h) HIPCHK(hipStreamSynchronize(h->stream));
+    // groups of one size and one Hpsi state, batched launches on the
+    // ensemble stream, each group's operand tables in its own part of dtab
+    std::map<std::pair<int, int>, std::vector<int>> groups;
+    for (size_t r = 0; r < nreal; ++r) groups[{e->h[r]->N, e->h[r]->hstate}].push_back((int)r);
+    size_t tabb = 0;
+    for (auto& kv : groups) tabb += (ens_diag_tab_bytes(kv.second.size()) + 255) / 256 * 256;
+    if (tabb > e->ntab) {
+        if (e->dtab) hipFree(e->dtab);
+        e->dtab = nullptr;
+        e->ntab = 0;
+        HIPCHK(hipMalloc(&e->dtab, tabb));
+        e->ntab = tabb;
+    }
+    size_t toff = 0;
+    for (auto& kv : groups) {
+        std::vector<vk_gpe*> hs;
+        std::vector<double*> su, sp;
+        for (int r : kv.second) {
+            hs.push_back(e->h[r]);
+            su.push_back(e->dstage + offs[r]);
+            sp.push_back(e->dstage + offs[r] + 8);
+        }
+        char* tab = static_cast<char*>(e->dtab) + toff;
+        toff += (ens_diag_tab_bytes(hs.size()) + 255) / 256 * 256;
+        int rc = dispatch(kv.first.first, [&](auto nn) {
+            using C = CfgN<decltype(nn)::value>;
+            if (kv.first.second == 0) {
+                // Hpsi stale (a member in its second shape): IFFT_y(K0) on
+                // all panels in the single-field layout, as ensure_any does
+                std::vector<vk::BPair> pr;
+                for (auto h : hs) pr.push_back({h->K0, h->Hpsi});
+                HIPCHK(hipMemcpyAsync(tab, pr.data(), pr.size() * sizeof(vk::BPair), hipMemcpyHostToDevice, e->stream));
+                HIPCHK(hipStreamSynchronize(e->stream));
+                const size_t cl = (size_t)C::TC * C::COLRS * sizeof(double);
+                set_lds(vk::k_col_inv_b<C>, cl);
+                hipLaunchKernelGGL(vk::k_col_inv_b<C>, dim3(C::NP, (int)hs.size()), dim3(C::COLT), cl, e->stream,
+                                   (const vk::BPair*)tab, (const cplx*)hs[0]->tw, hs[0]->geo, 0, 0);
+                HIPCHK(hipStreamSynchronize(e->stream));   // before the table is rewritten
+                for (auto h : hs) h->hstate = 1;
+            }
+            return ens_diag_group<C>(hs, hs[0]->hstate == 2 ? 1 : 0, e->stream, tab, su, sp);
+        });
+        if (rc) return rc;
+        for (auto h : hs)
+            if (h->ens_var) e->synced = 0;   // Hpsi now in the single-field layout
+    }
+    HIPCHK(hipMemcpyAsync(e->hstage, e->dstage, need * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipGetLastError());
-    off = 0;
-    size_t so = 0;
     const double dg[4] = {};
-    for (size_t r = 0; r < e->h.size(); ++r) {
+    size_t so = 0;
+    for (size_t r = 0; r < nreal; ++r) {
         const vk_gpe* h = e->h[r];
         const size_t ns = 6 * (size_t)(h->nshell + 1);
-        diag_final(h, e->hstage + off, e->hstage + off + 8, dg, scal ? scal + 8 * r : nullptr, spec ? spec + so : nullptr,
-                   nullptr);
-        off += 8 + ns;
+        diag_final(h, e->hstage + offs[r], e->hstage + offs[r] + 8, dg, scal ? scal + 8 * r : nullptr,
+                   spec ? spec + so : nullptr, nullptr);
         so += ns;
     }
     return 0;

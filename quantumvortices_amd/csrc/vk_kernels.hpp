@@ -124,10 +124,9 @@ __device__ __forceinline__ cplx czero() { return mk(0.0, 0.0); }
 
 // x-space rows -> FFT along x -> H.  op 1: transform |X|^2 instead of X.
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd(const cplx* __restrict__ X, cplx* __restrict__ H,
-                                                     const cplx* __restrict__ tw, Geo g, int pstore, int op)
+__device__ __forceinline__ void row_fwd_body(int blk_, int nblk_, const cplx* __restrict__ X, cplx* __restrict__ H,
+                                             const cplx* __restrict__ tw, Geo g, int pstore, int op)
 {
-    const int blk_ = blockIdx.x, nblk_ = gridDim.x;
     VK_ROW_SETUP(C)
     cplx v[S];
     const cplx* xr = X + (size_t)y * N;
@@ -144,12 +143,19 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd(const cplx* __restri
     }
 }
 
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd(const cplx* __restrict__ X, cplx* __restrict__ H,
+                                                     const cplx* __restrict__ tw, Geo g, int pstore, int op)
+{
+    row_fwd_body<C>(blockIdx.x, gridDim.x, X, H, tw, g, pstore, op);
+}
+
 // H column -> FFT along y -> K.  op 1: dealias.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
-                                                     const cplx* __restrict__ tw, Geo g, int pmode, int op)
+__device__ __forceinline__ void col_fwd_body(int b, const cplx* __restrict__ H, cplx* __restrict__ K,
+                                             const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
-    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, pmode))
+    VK_COL_SETUP(C, panel_of_block(b, C::NP, g, pmode))
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = H[hoff<C>(p, t + T * s, c)];
@@ -165,13 +171,20 @@ __global__ void __launch_bounds__(C::COLT) VK_OCC k_col_fwd(const cplx* __restri
     }
 }
 
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
+                                                     const cplx* __restrict__ tw, Geo g, int pmode, int op)
+{
+    col_fwd_body<C>(blockIdx.x, H, K, tw, g, pmode, op);
+}
+
 // K row -> op -> unnormalised inverse FFT along y -> H.
 // op 0: as is; 1: dealias; 2: times i*ky.
 template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
-                                                     const cplx* __restrict__ tw, Geo g, int pmode, int op)
+__device__ __forceinline__ void col_inv_body(int b, const cplx* __restrict__ K, cplx* __restrict__ H,
+                                             const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
-    VK_COL_SETUP(C, panel_of_block(blockIdx.x, C::NP, g, pmode))
+    VK_COL_SETUP(C, panel_of_block(b, C::NP, g, pmode))
     cplx v[S];
     const int kxs = ksg(kx, N);
     const cplx* kr = K + (size_t)kx * N;
@@ -189,15 +202,20 @@ __global__ void __launch_bounds__(C::COLT) VK_OCC k_col_inv(const cplx* __restri
     for (int s = 0; s < S; ++s) H[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
 }
 
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
+                                                     const cplx* __restrict__ tw, Geo g, int pmode, int op)
+{
+    col_inv_body<C>(blockIdx.x, K, H, tw, g, pmode, op);
+}
+
 // H row -> op -> unnormalised inverse FFT along x -> times scale -> X.
 // pload 2: kx >= 0 panels stored, negative kx from Hermitian symmetry.
 // op 1: times i*kx before the transform.
 template <class C>
-__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv(const cplx* __restrict__ H, cplx* __restrict__ X,
-                                                     const cplx* __restrict__ tw, Geo g, int pload, int op,
-                                                     double scale)
+__device__ __forceinline__ void row_inv_body(int blk_, int nblk_, const cplx* __restrict__ H, cplx* __restrict__ X,
+                                             const cplx* __restrict__ tw, Geo g, int pload, int op, double scale)
 {
-    const int blk_ = blockIdx.x, nblk_ = gridDim.x;
     VK_ROW_SETUP(C)
     cplx v[S];
     const int Kp = TC * g.Pp;
@@ -230,6 +248,14 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv(const cplx* __restri
     cplx* xr = X + (size_t)y * N;
 #pragma unroll
     for (int s = 0; s < S; ++s) xr[t + T * s] = rmul(scale, conjc(v[s]));
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv(const cplx* __restrict__ H, cplx* __restrict__ X,
+                                                     const cplx* __restrict__ tw, Geo g, int pload, int op,
+                                                     double scale)
+{
+    row_inv_body<C>(blockIdx.x, gridDim.x, H, X, tw, g, pload, op, scale);
 }
 
 // zero the dealiased modes of a k-space field (all N^2 entries)
@@ -979,7 +1005,8 @@ __global__ void __launch_bounds__(C::COLT) VK_OCC k_a4(const cplx* __restrict__ 
 }
 
 // fixed-order reduction of nb partial records of nv doubles -> out[nv]
-__global__ void k_reduce_fixed(const double* __restrict__ part, int nb, int nv, double* __restrict__ out)
+__device__ __forceinline__ void reduce_fixed_body(const double* __restrict__ part, int nb, int nv,
+                                                  double* __restrict__ out)
 {
     __shared__ double sh[1024];
     for (int i = 0; i < nv; ++i) {
@@ -990,6 +1017,11 @@ __global__ void k_reduce_fixed(const double* __restrict__ part, int nb, int nv, 
         if (threadIdx.x == 0) out[i] = xx[0];
         __syncthreads();
     }
+}
+
+__global__ void k_reduce_fixed(const double* __restrict__ part, int nb, int nv, double* __restrict__ out)
+{
+    reduce_fixed_body(part, nb, nv, out);
 }
 
 // ----------------------------------------------------------------------------
@@ -1007,15 +1039,16 @@ struct DiagArgs {
 //   w = 2 alpha Im(psi* grad psi)/(|psi|+smlnum) -> W1, W2
 //   q = Re(psi* grad psi)/(|psi|+smlnum)          -> Q1, Q2   (:263-317)
 // sums: 0 momx 1 momy 2 egrad 3 echem 4 eintr 5 eppot 6 norm 7 unused
-__global__ void k_diag_x(const cplx* __restrict__ psi, const cplx* __restrict__ D1,
-                         const cplx* __restrict__ D2, const cplx* __restrict__ PR,
-                         const cplx* __restrict__ V, cplx* __restrict__ W1, cplx* __restrict__ W2,
-                         cplx* __restrict__ Q1, cplx* __restrict__ Q2, DiagArgs a, double* __restrict__ part)
+__device__ __forceinline__ void diag_x_body(int blk, int nblk, const cplx* __restrict__ psi,
+                                            const cplx* __restrict__ D1, const cplx* __restrict__ D2,
+                                            const cplx* __restrict__ PR, const cplx* __restrict__ V,
+                                            cplx* __restrict__ W1, cplx* __restrict__ W2, cplx* __restrict__ Q1,
+                                            cplx* __restrict__ Q2, DiagArgs a, double* __restrict__ part)
 {
     __shared__ double sh[8 * 16];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const size_t n2 = (size_t)a.N * a.N;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = blk * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)nblk * blockDim.x) {
         const cplx ps = psi[i], d1 = D1[i], d2 = D2[i], pr = PR[i];
         const cplx cp = conjc(ps);
         const cplx a1 = cmul(cp, d1), a2 = cmul(cp, d2);
@@ -1037,7 +1070,15 @@ __global__ void k_diag_x(const cplx* __restrict__ psi, const cplx* __restrict__ 
     }
     block_sum<8>(acc, sh);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 8; ++k) part[(size_t)blockIdx.x * 8 + k] = acc[k];
+        for (int k = 0; k < 8; ++k) part[(size_t)blk * 8 + k] = acc[k];
+}
+
+__global__ void k_diag_x(const cplx* __restrict__ psi, const cplx* __restrict__ D1,
+                         const cplx* __restrict__ D2, const cplx* __restrict__ PR,
+                         const cplx* __restrict__ V, cplx* __restrict__ W1, cplx* __restrict__ W2,
+                         cplx* __restrict__ Q1, cplx* __restrict__ Q2, DiagArgs a, double* __restrict__ part)
+{
+    diag_x_body(blockIdx.x, gridDim.x, psi, D1, D2, PR, V, W1, W2, Q1, Q2, a, part);
 }
 
 struct SpecArgs {
@@ -1048,14 +1089,13 @@ struct SpecArgs {
 // Shell binning: one workgroup per shell m over the list of retained modes
 // in that shell (idx = kx*N + ky), fixed-order reduction.
 // spec[(i)*(nshell+1) + m], i = 0..5 = spec_gp(m, i+1) (:394-399).
-__global__ void k_spectra(const int* __restrict__ modes, const int* __restrict__ offs,
-                          const cplx* __restrict__ KW1, const cplx* __restrict__ KW2,
-                          const cplx* __restrict__ KQ1, const cplx* __restrict__ KQ2,
-                          const cplx* __restrict__ KR, const cplx* __restrict__ KP, SpecArgs a,
-                          double* __restrict__ spec)
+__device__ __forceinline__ void spectra_body(int m, const int* __restrict__ modes, const int* __restrict__ offs,
+                                             const cplx* __restrict__ KW1, const cplx* __restrict__ KW2,
+                                             const cplx* __restrict__ KQ1, const cplx* __restrict__ KQ2,
+                                             const cplx* __restrict__ KR, const cplx* __restrict__ KP, SpecArgs a,
+                                             double* __restrict__ spec)
 {
     __shared__ double sh[6 * 16];
-    const int m = blockIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int j = offs[m] + threadIdx.x; j < offs[m + 1]; j += blockDim.x) {
         const int idx = modes[j];
@@ -1089,6 +1129,88 @@ __global__ void k_spectra(const int* __restrict__ modes, const int* __restrict__
     block_sum<6>(acc, sh);
     if (threadIdx.x == 0)
         for (int i = 0; i < 6; ++i) spec[(size_t)i * (a.nshell + 1) + m] = acc[i];
+}
+
+__global__ void k_spectra(const int* __restrict__ modes, const int* __restrict__ offs,
+                          const cplx* __restrict__ KW1, const cplx* __restrict__ KW2,
+                          const cplx* __restrict__ KQ1, const cplx* __restrict__ KQ2,
+                          const cplx* __restrict__ KR, const cplx* __restrict__ KP, SpecArgs a,
+                          double* __restrict__ spec)
+{
+    spectra_body(blockIdx.x, modes, offs, KW1, KW2, KQ1, KQ2, KR, KP, a, spec);
+}
+
+// ----------------------------------------------------------------------------
+// Batched forms for many fields of one size (the ensemble's diagnostics):
+// blockIdx.y selects the field's pointers, blockIdx.x is the block within
+// the field, as in the single-field launch
+// ----------------------------------------------------------------------------
+
+struct BPair {
+    const cplx* src;
+    cplx* dst;
+};
+
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd_b(const BPair* __restrict__ P, const cplx* __restrict__ tw,
+                                                       Geo g, int pstore, int op)
+{
+    const BPair q = P[blockIdx.y];
+    row_fwd_body<C>(blockIdx.x, gridDim.x, q.src, q.dst, tw, g, pstore, op);
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_fwd_b(const BPair* __restrict__ P, const cplx* __restrict__ tw,
+                                                       Geo g, int pmode, int op)
+{
+    const BPair q = P[blockIdx.y];
+    col_fwd_body<C>(blockIdx.x, q.src, q.dst, tw, g, pmode, op);
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_inv_b(const BPair* __restrict__ P, const cplx* __restrict__ tw,
+                                                       Geo g, int pmode, int op)
+{
+    const BPair q = P[blockIdx.y];
+    col_inv_body<C>(blockIdx.x, q.src, q.dst, tw, g, pmode, op);
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_inv_b(const BPair* __restrict__ P, const cplx* __restrict__ tw,
+                                                       Geo g, int pload, int op, double scale)
+{
+    const BPair q = P[blockIdx.y];
+    row_inv_body<C>(blockIdx.x, gridDim.x, q.src, q.dst, tw, g, pload, op, scale);
+}
+
+// one field's diagnostics operands
+struct DiagB {
+    const cplx *psi, *D1, *D2, *PR;
+    cplx *W1, *W2, *Q1, *Q2;
+    double* part;     // 8 partial sums per block
+    double* sums;     // 8 sums
+    double* spec;     // 6 (nshell + 1) spectra
+    const cplx *KW1, *KW2, *KQ1, *KQ2, *KR, *KP;
+    DiagArgs da;
+    SpecArgs sa;
+};
+
+__global__ void k_diag_x_b(const DiagB* __restrict__ D)
+{
+    const DiagB& d = D[blockIdx.y];
+    diag_x_body(blockIdx.x, gridDim.x, d.psi, d.D1, d.D2, d.PR, nullptr, d.W1, d.W2, d.Q1, d.Q2, d.da, d.part);
+}
+
+__global__ void k_reduce_fixed_b(const DiagB* __restrict__ D, int nb)
+{
+    const DiagB& d = D[blockIdx.x];
+    reduce_fixed_body(d.part, nb, 8, d.sums);
+}
+
+__global__ void k_spectra_b(const int* __restrict__ modes, const int* __restrict__ offs, const DiagB* __restrict__ D)
+{
+    const DiagB& d = D[blockIdx.y];
+    spectra_body(blockIdx.x, modes, offs, d.KW1, d.KW2, d.KQ1, d.KQ2, d.KR, d.KP, d.sa, d.spec);
 }
 
 // ----------------------------------------------------------------------------
