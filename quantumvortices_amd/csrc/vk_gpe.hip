@@ -2455,7 +2455,8 @@ int vk_ens_autotune(vk_ens* e, int nsteps, double* ms_out)
             if (m != e->mode) e->synced = 0;
             e->mode = m;
             e->use_graph = gph;
-            if ((rc = ens_steps_async(e, 1))) return rc;   // warm (and capture)
+            // warm: capture, graph upload on the first replays
+            if ((rc = ens_steps_async(e, 3))) return rc;
             HIPCHK(hipEventRecord(a, e->stream));
             if ((rc = ens_steps_async(e, nsteps))) return rc;
             HIPCHK(hipEventRecord(b, e->stream));
