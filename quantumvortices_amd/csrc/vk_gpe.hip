@@ -2156,8 +2156,12 @@ int ens_step_launch(vk_ens* e)
         A.bar = e->dbar;
         A.err = e->derr;
         A.limit = e->plimit;
+        // no more workgroups than the largest pass has blocks: a small
+        // ensemble's barriers then wait for few arrivals
+        const int most = std::max(base[0], std::max(base[1], base[2]));
+        const int grid = std::min(e->pgrid, (most + 7) / 8 * 8);
         hipLaunchKernelGGL(vk::k_zero_u32, dim3(1), dim3(64), 0, e->stream, e->dbar, vk::kEnsStepBarriers);
-        hipLaunchKernelGGL(vk::k_ens_step, dim3(e->pgrid), dim3(256), e->plds, e->stream, (const vk::EnsReal*)e->dR, A);
+        hipLaunchKernelGGL(vk::k_ens_step, dim3(grid), dim3(256), e->plds, e->stream, (const vk::EnsReal*)e->dR, A);
         return 0;
     }
     struct Grp { int* m[3]; int* o[3]; int nb[3]; };
