@@ -1650,7 +1650,11 @@ int vk_gpe_autotune_rows(vk_gpe* h, int nsteps, int* choice, double* ms_out)
     h->p1pf = b1 == 2;
     h->tr1_p1 = b1 == 1;
     h->tr1_p3 = b3;
-    if (!rc) rc = copy(false);
+    // the starting state back, also after a failed trial (whose error code
+    // is the one returned)
+    hipStreamSynchronize(h->stream);
+    const int rc2 = copy(false);
+    if (!rc) rc = rc2;
     release();
     if (rc) return rc;
     h->hstate = 2;
