@@ -227,8 +227,8 @@ PASS_KERNEL = {"p1_rho_row": "vk::k_p1x<4096, TR=1 >", "p2_dealias_col": "vk::k_
 
 
 # row-pass variants the traffic summary was taken with (P1 one row per
-# workgroup, P3 two rows)
-PROFILED_ROWS = (1, 0)
+# workgroup, P3 one row)
+PROFILED_ROWS = (1, 1)
 ROW_P1_NAMES = ["P1 TR rows per workgroup", "P1 one row per workgroup", "P1 persistent with LDS prefetch"]
 ROW_P3_NAMES = ["P3 TR rows per workgroup", "P3 one row per workgroup"]
 

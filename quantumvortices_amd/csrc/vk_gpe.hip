@@ -269,15 +269,22 @@ void col_launch(vk_gpe* h, int kind, int nblocks, Kern k, A... args)
     ev_end(h, kind);
 }
 
-// P2 columns per workgroup.  P2 covers only the kx >= 0 panels: at 4096^2
-// its 683 two-column workgroups fill 512 slots (2 per CU) 1.33 times, and
-// one column per workgroup (1366 in fuller waves) is 4% faster; at 2048^2
-// the same split is 10% slower (16-byte row pieces of 4-column panels).
-// Results are bitwise the same either way.
+// P2 columns per workgroup.  P2 covers only the kx >= 0 panels.  At
+// 4096^2 one column per workgroup (1366 workgroups) is the fastest: with
+// 2-column panels 4% faster than whole panels, with 4-column panels
+// 0.0697 against 0.0707 ms for two columns; at 2048^2 one column is 10%
+// slower than whole panels.  Results are bitwise the same either way.
 #ifndef VK_P2_COLS_4096
 #define VK_P2_COLS_4096 1
 #endif
 template <class C> constexpr int p2_cols() { return C::N == 4096 ? VK_P2_COLS_4096 : C::TC; }
+// P4 columns per workgroup at 4096^2 (0: whole panels).  Two of the four
+// columns of a panel in a 512-thread workgroup: 0.170 ms against 0.189 for
+// whole panels (1024 threads, one workgroup per CU) and 0.193 for one.
+#ifndef VK_P4_COLS_4096
+#define VK_P4_COLS_4096 2
+#endif
+template <class C> constexpr int p4_cols() { return C::N == 4096 && VK_P4_COLS_4096 > 0 ? VK_P4_COLS_4096 : C::TC; }
 
 // column pass with G columns per workgroup over npan panels
 template <class C, int G, class Kern, class... A>
@@ -630,10 +637,11 @@ int rk4_step(vk_gpe* h)
             a.cstep = stage == 3 ? dt : dt * 0.5;
             a.r1 = (dt * oney6) / (dt * 0.5);
             a.r3 = (dt * oney6 * 2.0) / dt;
-            auto k4 = stage == 1 ? vk::k_p4<C, 1> : stage == 2 ? vk::k_p4<C, 2> : stage == 3 ? vk::k_p4<C, 3>
-                                                                                        : vk::k_p4<C, 4>;
-            col_launch<C>(h, PK_P4, h->geo.Pp + h->geo.Pn, k4, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
-                          h->K2, (const cplx*)h->tw, h->geo, a);
+            constexpr int G4 = p4_cols<C>();
+            auto k4 = stage == 1 ? vk::k_p4<C, 1, G4> : stage == 2 ? vk::k_p4<C, 2, G4>
+                                                      : stage == 3 ? vk::k_p4<C, 3, G4> : vk::k_p4<C, 4, G4>;
+            colg_launch<C, G4>(h, PK_P4, h->geo.Pp + h->geo.Pn, k4, (const cplx*)h->HN, h->Hpsi, h->K0, h->K1,
+                               h->K2, (const cplx*)h->tw, h->geo, a);
         }
         if (rs) join_particles(h);
         h->kv_valid = 0;
@@ -947,11 +955,12 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         const char* t1 = std::getenv("VK_P1_TR1");
         const char* t3 = std::getenv("VK_P3_TR1");
         // Measured defaults: one row per workgroup in P1 and P3 at 2048^2
-        // (13% faster per step than four rows); at 4096^2 one row in P1 (6%
-        // faster) and two in P3.  VK_P1_TR1 / VK_P3_TR1 = 0 or 1 override;
-        // vk_gpe_autotune_rows times every pair on the device at hand.
+        // and 4096^2 (both with 4-column panels: 64-byte row pieces); 13%
+        // faster per step than four rows at 2048^2, 3% than two at 4096^2.
+        // VK_P1_TR1 / VK_P3_TR1 = 0 or 1 override; vk_gpe_autotune_rows
+        // times every pair on the device at hand.
         h->tr1_p1 = t1 ? t1[0] == '1' : (N == 4096 || N == 2048);
-        h->tr1_p3 = t3 ? t3[0] == '1' : N == 2048;
+        h->tr1_p3 = t3 ? t3[0] == '1' : (N == 4096 || N == 2048);
     }
     // psi rows: the single-field RK4/ARGLE path at N >= VK_XPSI_MIN_N and the
     // ensemble kernels (which also step single fields of N <= 1024)

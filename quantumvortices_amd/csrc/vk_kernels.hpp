@@ -876,13 +876,15 @@ struct RKArgs {
 // of kpsi).  The stage is a compile-time constant, and the K accesses of
 // the dealiased modes go to the out-of-range offset (loads read zero,
 // stores are dropped), so the loads of all points are issued back to back.
-template <class C, int STAGE>
+// G < TC: G of a panel's TC columns per workgroup (sibling workgroups of a
+// panel on one XCD, as in P2).
+template <class C, int STAGE, int G = C::TC>
 __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
                                                 cplx* __restrict__ K2, const cplx* __restrict__ tw,
                                                 Geo g, RKArgs a)
 {
-    VK_COL_SETUP(C, panel_of_block(blk_, C::NP, g, 1))
+    VK_COL_SETUP_G(C, G, blk_, panel_of_block(pb_, C::NP, g, 1), g.Pp + g.Pn)
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
@@ -921,13 +923,13 @@ __device__ __forceinline__ void p4_body(int blk_, int nblk_, const cplx* __restr
     for (int s = 0; s < S; ++s) Hpsi[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
 }
 
-template <class C, int STAGE>
-__global__ void __launch_bounds__(C::COLT) VK_P4OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+template <class C, int STAGE, int G = C::TC>
+__global__ void __launch_bounds__(G * C::T) VK_P4OCC k_p4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, cplx* __restrict__ K1,
                                                 cplx* __restrict__ K2, const cplx* __restrict__ tw,
                                                 Geo g, RKArgs a)
 {
-    p4_body<C, STAGE>(blockIdx.x, gridDim.x, HN, Hpsi, K0, K1, K2, tw, g, a);
+    p4_body<C, STAGE, G>(blockIdx.x, gridDim.x, HN, Hpsi, K0, K1, K2, tw, g, a);
 }
 
 struct ArgleArgs {

@@ -1,15 +1,16 @@
 // Attainable HBM bandwidth of the passes' access patterns, measured on the
 // device: the ceilings the pass kernels' roofline fractions are read
 // against.  Each pattern moves one 4096^2 complex128 field (256 MiB) in and
-// one out, with no arithmetic:
+// one out, with no arithmetic, over the passes' panel layout
+// H[(p*N + y)*TC + c] with TC = Cfg<4096>::TC:
 //   0  contiguous copy;
-//   1  row-pass pattern: a 512-thread workgroup per pair of rows gathers the
-//      rows' 64-byte pieces of every 2-column panel H[(p*N + y)*2 + c] and
-//      writes the two rows contiguously (the panel layout of the passes);
-//   2  column-pass pattern: a 512-thread workgroup copies one panel, two
-//      columns of N points (P4);
-//   3  the same with one column per 256-thread workgroup, sibling columns
-//      of a panel on one XCD (P2).
+//   1  row-pass pattern: a 256-thread workgroup per row gathers the row's
+//      pieces (TC points) of every panel and writes the row contiguously
+//      (P1 and P3 with one row per workgroup);
+//   2  column-pass pattern: a workgroup of 256 threads per column copies
+//      the P4 share of a panel (the P4 columns per workgroup), sibling
+//      workgroups of a panel on one XCD;
+//   3  the same with one column per 256-thread workgroup (P2).
 #include "vk_cfg.hpp"
 #include "vikshobha.h"
 
@@ -20,16 +21,20 @@ int vk_set_error(int code, const std::string& msg);
 namespace {
 
 using vk::cplx;
-constexpr int PN = 4096, PTC = 2;
+constexpr int PN = 4096, PTC = vk::Cfg<4096>::TC;
+#ifndef VK_P4_COLS_4096
+#define VK_P4_COLS_4096 2
+#endif
+constexpr int PG4 = VK_P4_COLS_4096 > 0 ? VK_P4_COLS_4096 : PTC;
 
 __global__ void k_pat_copy(const cplx* __restrict__ a, cplx* __restrict__ b, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
-__global__ void __launch_bounds__(512) k_pat_row(const cplx* __restrict__ H, cplx* __restrict__ X)
+__global__ void __launch_bounds__(256) k_pat_row(const cplx* __restrict__ H, cplx* __restrict__ X)
 {
-    constexpr int T = 256, S = 16, TR = 2;
+    constexpr int T = 256, S = 16, TR = 1;
     const int u = threadIdx.x;
     const int q = u % (TR * PTC), rr = q / PTC, c = q % PTC, pidx0 = u / (TR * PTC);
     const int y = vk::xcd_remap_blk(blockIdx.x, gridDim.x) * TR + rr;
@@ -40,7 +45,7 @@ __global__ void __launch_bounds__(512) k_pat_row(const cplx* __restrict__ H, cpl
     for (int s = 0; s < S; ++s) X[(size_t)y * PN + PTC * pidx0 + c + T * s] = v[s];
 }
 
-// G columns of each 2-column panel per workgroup
+// G columns of each TC-column panel per workgroup
 template <int G>
 __global__ void __launch_bounds__(G * 256) k_pat_col(const cplx* __restrict__ H, cplx* __restrict__ K)
 {
@@ -82,9 +87,9 @@ extern "C" int vk_pattern_bandwidth(int reps, double* gbps, int n)
         auto launch = [&]() {
             switch (k) {
             case 0: hipLaunchKernelGGL(k_pat_copy, dim3(8192), dim3(256), 0, 0, a, b, cnt); break;
-            case 1: hipLaunchKernelGGL(k_pat_row, dim3(PN / 2), dim3(512), 0, 0, a, b); break;
-            case 2: hipLaunchKernelGGL(k_pat_col<2>, dim3(NP), dim3(512), 0, 0, a, b); break;
-            default: hipLaunchKernelGGL(k_pat_col<1>, dim3(NP * 2), dim3(256), 0, 0, a, b); break;
+            case 1: hipLaunchKernelGGL(k_pat_row, dim3(PN), dim3(256), 0, 0, a, b); break;
+            case 2: hipLaunchKernelGGL(k_pat_col<PG4>, dim3(NP * (PTC / PG4)), dim3(PG4 * 256), 0, 0, a, b); break;
+            default: hipLaunchKernelGGL(k_pat_col<1>, dim3(NP * PTC), dim3(256), 0, 0, a, b); break;
             }
         };
         for (int i = 0; i < 2; ++i) launch();
