@@ -223,7 +223,7 @@ def ensemble_bench(G, torch, steps, warmup):
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk4_4096_traffic.txt")
 PASS_KERNEL = {"p1_rho_row": "vk::k_p1x<4096, TR=1 >", "p2_dealias_col": "vk::k_p2<4096, 1, 0>",
-               "p3_nonlin_row": "vk::k_p3x<4096, 6>", "p4_rk4_col": "vk::k_p4<4096, avg>"}
+               "p3_nonlin_row": "vk::k_p3x<4096, TR=1, 6>", "p4_rk4_col": "vk::k_p4<4096, 2, avg>"}
 
 
 # row-pass variants the traffic summary was taken with (P1 one row per

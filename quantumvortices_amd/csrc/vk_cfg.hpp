@@ -53,8 +53,10 @@ template <> struct Cfg<2048> : CfgT<2048, 16, 4, 4, true> {};
 // 4-column panels: the row passes gather 64-byte row pieces with one row
 // per workgroup (P1 0.152 -> 0.139 ms, P3 0.186 -> 0.163 ms) and the column
 // passes take two of a panel's columns per workgroup (P4 0.151 -> 0.170
-// ms); a step is 1% faster with particles, 2.8% without, and the fields
-// are bitwise the same (the layout moves data, the arithmetic is unchanged)
+// ms).  Per step, against 2-column panels on the same device: 2-3% faster
+// without particles, within 1% with them (+0.9% and -0.4% on two
+// devices).  The fields are bitwise the same: the layout moves data, the
+// arithmetic is unchanged.
 #ifndef VK4096_TC
 #define VK4096_TC 4
 #endif
