@@ -726,11 +726,14 @@ int argle_step(vk_gpe* h, int want_res, const double* noise = nullptr)
             a.sdt = std::sqrt(h->p.dt);
             a.ktrunc2 = h->sg_ktrunc > 0.0 ? h->sg_ktrunc * h->sg_ktrunc : 0.0;
         }
-        col_launch<C>(h, PK_ARG4, h->NP, vk::k_a4<C>, (const cplx*)h->HN, h->Hpsi, h->K0, (const cplx*)h->tw, a,
-                      h->part);
+        // the columns per workgroup of P4 (two of four at 4096^2)
+        constexpr int GA = p4_cols<C>();
+        static_assert(GA == C::TC || C::NP % 8 == 0, "column groups need whole groups of 8 panels");
+        colg_launch<C, GA>(h, PK_ARG4, h->NP, vk::k_a4<C, GA>, (const cplx*)h->HN, h->Hpsi, h->K0,
+                           (const cplx*)h->tw, a, h->part);
         if (want_res)
-            hipLaunchKernelGGL(vk::k_reduce_fixed, dim3(1), dim3(256), 0, h->stream, (const double*)h->part, h->NP, 2,
-                               h->dsum);
+            hipLaunchKernelGGL(vk::k_reduce_fixed, dim3(1), dim3(256), 0, h->stream, (const double*)h->part,
+                               h->NP * (C::TC / GA), 2, h->dsum);
         return 0;
     });
 }
@@ -985,7 +988,8 @@ int vk_gpe_create(vk_gpe** out, const vk_gpe_params* p)
         if (hipMemcpy(h->tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(VK_ERR_HIP, "twiddle upload"));
     }
-    if (hipMalloc(&h->part, sizeof(double) * 2 * h->NP) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc part"));
+    // residual partials: one pair per ARGLE column workgroup (at most one per column)
+    if (hipMalloc(&h->part, sizeof(double) * 2 * N) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc part"));
     if (hipMalloc(&h->dsum, sizeof(double) * 8) != hipSuccess) return cleanup(fail(VK_ERR_HIP, "hipMalloc dsum"));
     if (p->np > 0) {
         const int np = p->np;

@@ -947,13 +947,15 @@ struct ArgleArgs {
 // ARGLE A4 (column, all panels): forward along y -> N-hat; implicit update
 // kpsi = (kpsi + dt*(N-hat + omega*kpsi))/(1 + alpha*k^2*dt) on every mode
 // (src/gpe.f90:195-197, no dealiasing of the result); residual partial
-// sums; inverse along y of the new kpsi -> Hpsi.
-template <class C>
+// sums; inverse along y of the new kpsi -> Hpsi.  G < TC: G of a panel's
+// columns per workgroup (the host keeps NP a multiple of 8 then, so that
+// every workgroup writes its residual partials).
+template <class C, int G = C::TC>
 __device__ __forceinline__ void a4_body(int blk_, int nblk_, const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, const cplx* __restrict__ tw,
                                                 ArgleArgs a, double* __restrict__ part)
 {
-    VK_COL_SETUP(C, blk_)
+    VK_COL_SETUP_G(C, G, blk_, pb_, C::NP)
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = HN[hoff<C>(p, t + T * s, c)];
@@ -998,12 +1000,12 @@ __device__ __forceinline__ void a4_body(int blk_, int nblk_, const cplx* __restr
     }
 }
 
-template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
+template <class C, int G = C::TC>
+__global__ void __launch_bounds__(G * C::T) VK_OCC k_a4(const cplx* __restrict__ HN, cplx* __restrict__ Hpsi,
                                                 cplx* __restrict__ K0, const cplx* __restrict__ tw,
                                                 ArgleArgs a, double* __restrict__ part)
 {
-    a4_body<C>(blockIdx.x, gridDim.x, HN, Hpsi, K0, tw, a, part);
+    a4_body<C, G>(blockIdx.x, gridDim.x, HN, Hpsi, K0, tw, a, part);
 }
 
 // fixed-order reduction of nb partial records of nv doubles -> out[nv]
