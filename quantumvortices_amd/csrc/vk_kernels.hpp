@@ -40,12 +40,18 @@
 #endif
 namespace vk { constexpr bool kP3FregPart = VK_P3_FREG_PART != 0; }
 // P3 on the psi-row path without particles (PM = 0) at 4 waves per SIMD
-// (138 -> 128 VGPRs, two workgroups per CU): +0.8% pure-GPE steps/s
+// (<= 128 VGPRs): +0.8% pure-GPE steps/s with two rows per workgroup.  With
+// particles the compiler's choice (231 VGPRs with one row per workgroup,
+// two workgroups per CU) is kept: at 3 waves per SIMD (168 VGPRs, 188 B of
+// spills) P3 takes 0.243 instead of 0.163 ms.
 #ifndef VK_P3_WAVES
 #define VK_P3_WAVES 4
 #endif
 #if VK_P3_WAVES > 0
-#define VK_P3OCC(PM) __attribute__((amdgpu_waves_per_eu((PM) == 0 ? VK_P3_WAVES : 1)))
+#ifndef VK_P3_PART_WAVES
+#define VK_P3_PART_WAVES 1
+#endif
+#define VK_P3OCC(PM) __attribute__((amdgpu_waves_per_eu((PM) == 0 ? VK_P3_WAVES : VK_P3_PART_WAVES)))
 #else
 #define VK_P3OCC(PM) VK_OCC
 #endif
