@@ -5,8 +5,9 @@ seeded particle positions (seed 4), V0 = 10 g, d = 2 xi (the real-space
 potential path inside the nonlinear row pass), mass 5x the displaced fluid.
 Two RK4 steps on the GPU, with the persistent prefetching P1 and with the
 plain P1 (VK_P1_PREFETCH=1 / 0), and with the default row-pass shapes (one
-row per workgroup in P1, two in P3), two rows in both, or one row in both
-(VK_P1_TR1, VK_P3_TR1), each against the same two oracle steps
+row per workgroup in P1 and P3, 4-column panels), two rows in both, or one
+row in both set explicitly (VK_P1_TR1, VK_P3_TR1), each against the same two
+oracle steps
 (src/gpe.f90:310-494, src/gpe_particle.f90:186-531), then the diagnostics
 (energies, six spectra, particle forces and particle energies,
 src/diagnostics.f90:54-533).  The oracle runs on the host's cores."""
