@@ -367,7 +367,9 @@ int fwd2d(vk_gpe* h, const cplx* X, cplx* Hs, cplx* K, int op_row, int op_col)
     return dispatch(h->N, [&](auto n) {
         using C = CfgN<decltype(n)::value>;
         row_launch<C>(h, PK_OTHER, vk::k_row_fwd<C>, X, Hs, (const cplx*)h->tw, h->geo, 0, op_row);
-        col_launch<C>(h, PK_OTHER, h->NP, vk::k_col_fwd<C>, (const cplx*)Hs, K, (const cplx*)h->tw, h->geo, 0, op_col);
+        constexpr int G = p4_cols<C>();
+        colg_launch<C, G>(h, PK_OTHER, h->NP, vk::k_col_fwd<C, G>, (const cplx*)Hs, K, (const cplx*)h->tw, h->geo, 0,
+                          op_col);
         return 0;
     });
 }
@@ -380,8 +382,9 @@ int fwd2d_half(vk_gpe* h, const cplx* X, cplx* Hs, cplx* K, int op_row, int op_c
     return dispatch(h->N, [&](auto n) {
         using C = CfgN<decltype(n)::value>;
         row_launch<C>(h, PK_OTHER, vk::k_row_fwd<C>, X, Hs, (const cplx*)h->tw, h->geo, 2, op_row);
-        col_launch<C>(h, PK_OTHER, h->geo.Pp, vk::k_col_fwd<C>, (const cplx*)Hs, K, (const cplx*)h->tw, h->geo, 2,
-                      op_col);
+        constexpr int G = p4_cols<C>();
+        colg_launch<C, G>(h, PK_OTHER, h->geo.Pp, vk::k_col_fwd<C, G>, (const cplx*)Hs, K, (const cplx*)h->tw, h->geo, 2,
+                          op_col);
         return 0;
     });
 }
@@ -390,7 +393,9 @@ int col_inv(vk_gpe* h, const cplx* K, cplx* Hs, int pmode, int op)
 {
     return dispatch(h->N, [&](auto n) {
         using C = CfgN<decltype(n)::value>;
-        col_launch<C>(h, PK_OTHER, ncols(h, pmode), vk::k_col_inv<C>, K, Hs, (const cplx*)h->tw, h->geo, pmode, op);
+        constexpr int G = p4_cols<C>();
+        colg_launch<C, G>(h, PK_OTHER, ncols(h, pmode), vk::k_col_inv<C, G>, K, Hs, (const cplx*)h->tw, h->geo, pmode,
+                          op);
         return 0;
     });
 }
@@ -689,8 +694,10 @@ int argle_step(vk_gpe* h, int want_res, const double* noise = nullptr)
             h->kv_valid = 1;
         }
         const bool adv = h->adv_tg || h->adv_field || h->adv_vx != 0.0 || h->adv_vy != 0.0;
-        if (adv) col_launch<C>(h, PK_OTHER, h->NP, vk::k_col_inv<C>, (const cplx*)h->K0, h->Hdy, (const cplx*)h->tw,
-                               h->geo, 0, 2);   // i ky kpsi -> Hdy (all columns)
+        constexpr int GC = p4_cols<C>();
+        if (adv)   // i ky kpsi -> Hdy (all columns)
+            colg_launch<C, GC>(h, PK_OTHER, h->NP, vk::k_col_inv<C, GC>, (const cplx*)h->K0, h->Hdy,
+                               (const cplx*)h->tw, h->geo, 0, 2);
         if (h->xpsi)
             row_launch<C>(h, PK_P1, vk::k_p1x<C>, (const cplx*)h->Hpsi, h->Hrho, (const cplx*)h->tw, h->geo, 0, sc,
                           h->Xs);

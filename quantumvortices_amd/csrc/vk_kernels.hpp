@@ -157,11 +157,14 @@ __global__ void __launch_bounds__(C::ROWT) VK_OCC k_row_fwd(const cplx* __restri
 }
 
 // H column -> FFT along y -> K.  op 1: dealias.
-template <class C>
+// panels a column pass over pmode covers (0 all, 1 retained, 2 kx >= 0)
+#define VK_NPAN(C, g, pmode) ((pmode) == 0 ? C::NP : ((pmode) == 2 ? (g).Pp : (g).Pp + (g).Pn))
+
+template <class C, int G = C::TC>
 __device__ __forceinline__ void col_fwd_body(int b, const cplx* __restrict__ H, cplx* __restrict__ K,
                                              const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
-    VK_COL_SETUP(C, panel_of_block(b, C::NP, g, pmode))
+    VK_COL_SETUP_G(C, G, b, panel_of_block(pb_, C::NP, g, pmode), VK_NPAN(C, g, pmode))
     cplx v[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) v[s] = H[hoff<C>(p, t + T * s, c)];
@@ -177,20 +180,20 @@ __device__ __forceinline__ void col_fwd_body(int b, const cplx* __restrict__ H, 
     }
 }
 
-template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
+template <class C, int G = C::TC>
+__global__ void __launch_bounds__(G * C::T) VK_OCC k_col_fwd(const cplx* __restrict__ H, cplx* __restrict__ K,
                                                      const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
-    col_fwd_body<C>(blockIdx.x, H, K, tw, g, pmode, op);
+    col_fwd_body<C, G>(blockIdx.x, H, K, tw, g, pmode, op);
 }
 
 // K row -> op -> unnormalised inverse FFT along y -> H.
 // op 0: as is; 1: dealias; 2: times i*ky.
-template <class C>
+template <class C, int G = C::TC>
 __device__ __forceinline__ void col_inv_body(int b, const cplx* __restrict__ K, cplx* __restrict__ H,
                                              const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
-    VK_COL_SETUP(C, panel_of_block(b, C::NP, g, pmode))
+    VK_COL_SETUP_G(C, G, b, panel_of_block(pb_, C::NP, g, pmode), VK_NPAN(C, g, pmode))
     cplx v[S];
     const int kxs = ksg(kx, N);
     const cplx* kr = K + (size_t)kx * N;
@@ -208,11 +211,11 @@ __device__ __forceinline__ void col_inv_body(int b, const cplx* __restrict__ K, 
     for (int s = 0; s < S; ++s) H[hoff<C>(p, t + T * s, c)] = conjc(v[s]);
 }
 
-template <class C>
-__global__ void __launch_bounds__(C::COLT) VK_OCC k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
+template <class C, int G = C::TC>
+__global__ void __launch_bounds__(G * C::T) VK_OCC k_col_inv(const cplx* __restrict__ K, cplx* __restrict__ H,
                                                      const cplx* __restrict__ tw, Geo g, int pmode, int op)
 {
-    col_inv_body<C>(blockIdx.x, K, H, tw, g, pmode, op);
+    col_inv_body<C, G>(blockIdx.x, K, H, tw, g, pmode, op);
 }
 
 // H row -> op -> unnormalised inverse FFT along x -> times scale -> X.
