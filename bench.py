@@ -511,8 +511,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        # the host cores this process may use, at most the 16 a one-GPU box
-        # grants a job (OMP_NUM_THREADS there)
+        # the host cores this process may use, capped by OMP_NUM_THREADS
+        # (16 when unset)
         share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
         threads = args.cpu_threads or max(1, min(len(os.sched_getaffinity(0)), share))
         log("bench: CPU baseline (oracle) with %d threads" % threads)

@@ -114,14 +114,16 @@ def test_twiddles(oracle_mod):
 
 # --------------------------------------------------- grid and constants -----
 
-SURVEY_NSHELL = {128: 91, 256: 181, 512: 362, 1024: 724, 4096: 2896}
+# nshell = int(1.414 N / 2) + 1 with 1.414 N in single precision
+# (src/mod_grid.f90:63-101), worked out by hand
+NSHELL_EXPECTED = {128: 91, 256: 181, 512: 362, 1024: 724, 4096: 2896}
 
 
-@pytest.mark.parametrize("N", sorted(SURVEY_NSHELL))
+@pytest.mark.parametrize("N", sorted(NSHELL_EXPECTED))
 def test_grid_constants(oracle_mod, N):
     O = oracle_mod
     p = O.make_params(N, 1e-3, 2.5)
-    assert p.nshell == SURVEY_NSHELL[N]
+    assert p.nshell == NSHELL_EXPECTED[N]
     assert abs(p.kalias - N / 3.0) == 0.0
     assert abs(p.dx - 2 * math.pi / N) < 1e-18
     assert abs(p.alpha - p.xi ** 2) < 1e-15 * p.xi ** 2      # alpha = xi^2 for g = 1, rho = 1
