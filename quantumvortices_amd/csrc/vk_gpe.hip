@@ -1743,6 +1743,7 @@ struct vk_ens {
     double* dstage = nullptr;   // the same on the device
     size_t nstage = 0;
     void* dtab = nullptr;       // operand tables of the batched diagnostics
+    void* htab = nullptr;       // their pinned host staging
     size_t ntab = 0;
     int synced = 0;
     std::map<int, int> var;                       // N -> shape variant
@@ -1973,7 +1974,9 @@ int col_inv_var(vk_gpe* h, int var)
 // the single-field pipeline (diag_launch) with blockIdx.y selecting the
 // member, so each member gets bit for bit its single-field result.  Member
 // i's sums go to sums[i] (8 doubles) and its spectra to spec[i].
-// tab: device memory for the operand tables, ens_diag_tab_bytes(hs.size()).
+// tab: device memory for the operand tables, ens_diag_tab_bytes(hs.size());
+// htab: pinned host memory of the same size, not rewritten before the
+// stream has passed the copy.
 constexpr int kDiagPairSteps = 16;
 size_t ens_diag_tab_bytes(size_t n)
 {
@@ -1981,7 +1984,7 @@ size_t ens_diag_tab_bytes(size_t n)
 }
 
 template <class C>
-int ens_diag_group(const std::vector<vk_gpe*>& hs, int pl, hipStream_t st, void* tab,
+int ens_diag_group(const std::vector<vk_gpe*>& hs, int pl, hipStream_t st, void* tab, void* htab,
                    const std::vector<double*>& sums, const std::vector<double*>& spec)
 {
     const int n = (int)hs.size();
@@ -1989,8 +1992,9 @@ int ens_diag_group(const std::vector<vk_gpe*>& hs, int pl, hipStream_t st, void*
     const Geo g = h0->geo;
     const cplx* tw = h0->tw;   // the same table in every member of this size
     const double sc = scale_of(h0);
-    std::vector<vk::BPair> pairs((size_t)kDiagPairSteps * n);
-    std::vector<vk::DiagB> db(n);
+    const size_t dd_off = ((kDiagPairSteps * n * sizeof(vk::BPair) + 255) / 256) * 256;
+    vk::BPair* pairs = static_cast<vk::BPair*>(htab);
+    vk::DiagB* db = reinterpret_cast<vk::DiagB*>(static_cast<char*>(htab) + dd_off);
     for (int i = 0; i < n; ++i) {
         vk_gpe* h = hs[i];
         cplx** X = h->X;
@@ -2015,11 +2019,8 @@ int ens_diag_group(const std::vector<vk_gpe*>& hs, int pl, hipStream_t st, void*
     }
     char* base = static_cast<char*>(tab);
     vk::BPair* dp = reinterpret_cast<vk::BPair*>(base);
-    vk::DiagB* dd = reinterpret_cast<vk::DiagB*>(base + ((kDiagPairSteps * n * sizeof(vk::BPair) + 255) / 256) * 256);
-    // pageable sources: the copies complete before the calls return
-    HIPCHK(hipMemcpyAsync(dp, pairs.data(), pairs.size() * sizeof(vk::BPair), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dd, db.data(), db.size() * sizeof(vk::DiagB), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
+    vk::DiagB* dd = reinterpret_cast<vk::DiagB*>(base + dd_off);
+    HIPCHK(hipMemcpyAsync(tab, htab, ens_diag_tab_bytes(n), hipMemcpyHostToDevice, st));
     const size_t rl = (size_t)C::TR * C::ROWRS * sizeof(double), cl = (size_t)C::TC * C::COLRS * sizeof(double);
     set_lds(vk::k_row_inv_b<C>, rl);
     set_lds(vk::k_row_fwd_b<C>, rl);
@@ -2123,6 +2124,7 @@ void vk_ens_destroy(vk_ens* e)
     if (e->hstage) hipHostFree(e->hstage);
     if (e->dstage) hipFree(e->dstage);
     if (e->dtab) hipFree(e->dtab);
+    if (e->htab) hipHostFree(e->htab);
     for (auto h : e->h) vk_gpe_destroy(h);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -2344,13 +2346,20 @@ int vk_ens_diagnostics_all(vk_ens* e, double* scal, double* spec)
     // ensemble stream, each group's operand tables in its own part of dtab
     std::map<std::pair<int, int>, std::vector<int>> groups;
     for (size_t r = 0; r < nreal; ++r) groups[{e->h[r]->N, e->h[r]->hstate}].push_back((int)r);
+    // per group: the ensure table (one pair per member), then the group's
+    // tables; pinned staging in the same layout, so no copy waits
+    auto ens_bytes = [](size_t n) { return (n * sizeof(vk::BPair) + 255) / 256 * 256; };
     size_t tabb = 0;
-    for (auto& kv : groups) tabb += (ens_diag_tab_bytes(kv.second.size()) + 255) / 256 * 256;
+    for (auto& kv : groups)
+        tabb += ens_bytes(kv.second.size()) + (ens_diag_tab_bytes(kv.second.size()) + 255) / 256 * 256;
     if (tabb > e->ntab) {
         if (e->dtab) hipFree(e->dtab);
+        if (e->htab) hipHostFree(e->htab);
         e->dtab = nullptr;
+        e->htab = nullptr;
         e->ntab = 0;
         HIPCHK(hipMalloc(&e->dtab, tabb));
+        HIPCHK(hipHostMalloc(&e->htab, tabb));
         e->ntab = tabb;
     }
     size_t toff = 0;
@@ -2362,25 +2371,27 @@ int vk_ens_diagnostics_all(vk_ens* e, double* scal, double* spec)
             su.push_back(e->dstage + offs[r]);
             sp.push_back(e->dstage + offs[r] + 8);
         }
+        char* etab = static_cast<char*>(e->dtab) + toff;
+        char* ehtab = static_cast<char*>(e->htab) + toff;
+        toff += ens_bytes(hs.size());
         char* tab = static_cast<char*>(e->dtab) + toff;
+        char* htab = static_cast<char*>(e->htab) + toff;
         toff += (ens_diag_tab_bytes(hs.size()) + 255) / 256 * 256;
         int rc = dispatch(kv.first.first, [&](auto nn) {
             using C = CfgN<decltype(nn)::value>;
             if (kv.first.second == 0) {
                 // Hpsi stale (a member in its second shape): IFFT_y(K0) on
                 // all panels in the single-field layout, as ensure_any does
-                std::vector<vk::BPair> pr;
-                for (auto h : hs) pr.push_back({h->K0, h->Hpsi});
-                HIPCHK(hipMemcpyAsync(tab, pr.data(), pr.size() * sizeof(vk::BPair), hipMemcpyHostToDevice, e->stream));
-                HIPCHK(hipStreamSynchronize(e->stream));
+                vk::BPair* pr = reinterpret_cast<vk::BPair*>(ehtab);
+                for (size_t i = 0; i < hs.size(); ++i) pr[i] = vk::BPair{hs[i]->K0, hs[i]->Hpsi};
+                HIPCHK(hipMemcpyAsync(etab, ehtab, hs.size() * sizeof(vk::BPair), hipMemcpyHostToDevice, e->stream));
                 const size_t cl = (size_t)C::TC * C::COLRS * sizeof(double);
                 set_lds(vk::k_col_inv_b<C>, cl);
                 hipLaunchKernelGGL(vk::k_col_inv_b<C>, dim3(C::NP, (int)hs.size()), dim3(C::COLT), cl, e->stream,
-                                   (const vk::BPair*)tab, (const cplx*)hs[0]->tw, hs[0]->geo, 0, 0);
-                HIPCHK(hipStreamSynchronize(e->stream));   // before the table is rewritten
+                                   (const vk::BPair*)etab, (const cplx*)hs[0]->tw, hs[0]->geo, 0, 0);
                 for (auto h : hs) h->hstate = 1;
             }
-            return ens_diag_group<C>(hs, hs[0]->hstate == 2 ? 1 : 0, e->stream, tab, su, sp);
+            return ens_diag_group<C>(hs, hs[0]->hstate == 2 ? 1 : 0, e->stream, tab, htab, su, sp);
         });
         if (rc) return rc;
         for (auto h : hs)
