@@ -176,6 +176,29 @@ def test_ensemble_384_mix(G):
     e.close()
 
 
+def test_ensemble_384_persistent(G):
+    """The 384-member mix stepped by the persistent launch (all resident
+    workgroups, grid barriers between the 16 passes), graph-replayed: sampled
+    members of every size equal their single-field runs bit for bit."""
+    say("case ensemble 384 members, persistent step")
+    sizes = [128] * 128 + [256] * 128 + [512] * 128
+    ps = [G.params_from_namelist(N, 1e-3, 2.5) for N in sizes]
+    ics = [G.random_vortices(2 * int(np.random.default_rng(r).integers(4, 33)), seed=r) for r in range(len(sizes))]
+    e = G.Ensemble(ps)
+    for r, (xs, ys, qs) in enumerate(ics):
+        e.set_ic_vortices(r, xs, ys, qs)
+    e.set_mode(3, True)
+    e.rk4(3)
+    for r in (0, 100, 127, 128, 200, 255, 256, 301, 383):
+        g = G.GPE(ps[r])
+        xs, ys, qs = ics[r]
+        g.set_ic_vortices(xs, ys, qs)
+        g.gpe_rk4(3)
+        assert np.array_equal(e.psi(r), g.psi), r
+        g.close()
+    e.close()
+
+
 def test_ensemble_shapes(G, oracle_mod):
     """Members of sizes 256..1024 in their second shape (8 points per
     thread, radix-8 stages, half-width panels): within roundoff of the
