@@ -143,6 +143,16 @@ __global__ void __launch_bounds__(256) VK_ENS_COCC k_ens_p4(const EnsReal* __res
     ens_tile_p4(R, map, off, blockIdx.x, stage);
 }
 
+// the same with the stage a compile-time constant: one stage's bodies per
+// kernel (fewer registers than the four stages behind a switch)
+template <int STAGE>
+__global__ void __launch_bounds__(256) VK_ENS_COCC k_ens_p4s(const EnsReal* __restrict__ R, const int* __restrict__ map,
+                                                   const int* __restrict__ off)
+{
+    VK_ENS_SETUP_B(blockIdx.x)
+    VK_ENS_SWITCH(e.N, (p4_body<C, STAGE>(lb, nb, e.HN, e.Hpsi, e.K0, e.K1, e.K2, e.tw, e.g, ens_rk_args(e, STAGE))))
+}
+
 // ---------------------------------------------------------------------------
 // Persistent step: one launch runs the 16 passes of an RK4 step for every
 // member, with a grid barrier between passes.  The blocks of a pass are

@@ -1856,7 +1856,24 @@ int make_single_table(vk_gpe* h)
     set_lds(vk::k_ens_p2, h->e1lds[1]);
     set_lds(vk::k_ens_p3, h->e1lds[2]);
     set_lds(vk::k_ens_p4, h->e1lds[3]);
+    for (auto k : {vk::k_ens_p4s<1>, vk::k_ens_p4s<2>, vk::k_ens_p4s<3>, vk::k_ens_p4s<4>}) set_lds(k, h->e1lds[3]);
     return 0;
+}
+
+// the ensemble P4 of one stage: a kernel per stage, or one kernel taking
+// the stage at run time (VK_ENS_P4_RUNTIME_STAGE=1)
+#ifndef VK_ENS_P4_RUNTIME_STAGE
+#define VK_ENS_P4_RUNTIME_STAGE 0
+#endif
+void ens_p4_launch(hipStream_t st, int nb, size_t lds, const vk::EnsReal* R, const int* m, const int* o, int stage)
+{
+    if (VK_ENS_P4_RUNTIME_STAGE) {
+        hipLaunchKernelGGL(vk::k_ens_p4, dim3(nb), dim3(256), lds, st, R, m, o, stage);
+        return;
+    }
+    auto k = stage == 1 ? vk::k_ens_p4s<1> : stage == 2 ? vk::k_ens_p4s<2> : stage == 3 ? vk::k_ens_p4s<3>
+                                                                                   : vk::k_ens_p4s<4>;
+    hipLaunchKernelGGL(k, dim3(nb), dim3(256), lds, st, R, m, o);
 }
 
 // launch the four passes of every stage over the given block maps
@@ -1871,7 +1888,7 @@ void ens_passes(hipStream_t st, const vk::EnsReal* R, int* const* m, int* const*
         if (th) ev_end(th, PK_P2), ev_begin(th);
         hipLaunchKernelGGL(vk::k_ens_p3, dim3(nb[0]), dim3(256), lds[2], st, R, (const int*)m[0], (const int*)o[0]);
         if (th) ev_end(th, PK_P3), ev_begin(th);
-        hipLaunchKernelGGL(vk::k_ens_p4, dim3(nb[2]), dim3(256), lds[3], st, R, (const int*)m[2], (const int*)o[2], stage);
+        ens_p4_launch(st, nb[2], lds[3], R, (const int*)m[2], (const int*)o[2], stage);
         if (th) ev_end(th, PK_P4);
     }
 }
@@ -2083,6 +2100,7 @@ int vk_ens_create(vk_ens** out, int nreal, const vk_gpe_params* params)
     set_lds(vk::k_ens_p2, e->lds[1]);
     set_lds(vk::k_ens_p3, e->lds[2]);
     set_lds(vk::k_ens_p4, e->lds[3]);
+    for (auto k : {vk::k_ens_p4s<1>, vk::k_ens_p4s<2>, vk::k_ens_p4s<3>, vk::k_ens_p4s<4>}) set_lds(k, e->lds[3]);
     for (int N : e->sizes) dispatch(N, [&](auto n) { preconfig_passes<CfgN<decltype(n)::value>>(); return 0; });
     // the persistent step: as many workgroups as the device holds at once
     {
@@ -2218,8 +2236,8 @@ int ens_step_launch(vk_ens* e)
             hipLaunchKernelGGL(vk::k_ens_p3, dim3(g.nb[0]), dim3(256), e->lds[2], e->stream,
                                (const vk::EnsReal*)e->dR, (const int*)g.m[0], (const int*)g.o[0]);
         for (auto& g : gs)
-            hipLaunchKernelGGL(vk::k_ens_p4, dim3(g.nb[2]), dim3(256), e->lds[3], e->stream,
-                               (const vk::EnsReal*)e->dR, (const int*)g.m[2], (const int*)g.o[2], stage);
+            ens_p4_launch(e->stream, g.nb[2], e->lds[3], (const vk::EnsReal*)e->dR, (const int*)g.m[2],
+                          (const int*)g.o[2], stage);
     }
     return 0;
 }
