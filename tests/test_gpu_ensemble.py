@@ -74,6 +74,10 @@ def test_ensemble_diagnostics_all(G):
     e = G.Ensemble(ps)
     for r, (xs, ys, qs) in enumerate(ics):
         e.set_ic_vortices(r, xs, ys, qs)
+    before = e.diagnostics_all()           # before any step: the initial fields
+    for r in range(len(sizes)):
+        scal, spec = e.diagnostics(r)
+        assert np.array_equal(before[r][0], scal) and np.array_equal(before[r][1], spec), r
     e.set_shape(512, 1)
     e.rk4(2)
     allr = e.diagnostics_all()
