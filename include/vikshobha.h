@@ -204,6 +204,14 @@ int vk_gpe_pass_times(vk_gpe* h, int nsteps, double* ms_out, int nkinds);
  * the particle state), keeps the fastest pair (choice[2] = {p1, p3}) and
  * returns ms/step in ms_out[2*p1 + p3] (6 entries, -1 where a variant does
  * not exist). */
+/* Transform shape of a single field of N in {256, 512, 1024} without
+ * particles, whose RK4 step runs the batched-ensemble kernels on one
+ * member: 0 (default; bitwise the result of a default ensemble member) or 1
+ * (8 points per thread, radix-8 stages, half-width panels: more, smaller
+ * workgroups, faster on one field; results agree to roundoff).  No
+ * reference counterpart. */
+int vk_gpe_set_shape(vk_gpe* h, int var);
+int vk_gpe_get_shape(vk_gpe* h, int* var);
 int vk_gpe_set_row_variants(vk_gpe* h, int p1, int p3);
 int vk_gpe_get_row_variants(vk_gpe* h, int* p1, int* p3);
 int vk_gpe_autotune_rows(vk_gpe* h, int nsteps, int* choice, double* ms_out);

@@ -71,6 +71,8 @@ SIGNATURES = {
     "vk_gpe_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "vk_gpe_pass_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_int]),
     "vk_gpe_set_row_variants": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "vk_gpe_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "vk_gpe_get_shape": (ctypes.c_int, [ctypes.c_void_p, ip]),
     "vk_gpe_get_row_variants": (ctypes.c_int, [ctypes.c_void_p, ip, ip]),
     "vk_gpe_autotune_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ip, dp]),
     "vk_pass_name": (ctypes.c_char_p, [ctypes.c_int]),

@@ -140,7 +140,8 @@ struct vk_gpe {
     cplx *HV = nullptr, *KV = nullptr, *Krho = nullptr, *Vhat = nullptr, *EX = nullptr, *EY = nullptr;
     double *fpart = nullptr, *pst = nullptr, *part = nullptr, *dsum = nullptr;
     unsigned char* phit = nullptr;   // real-space path: window flags per row block and particle
-    int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels
+    int hstate = 0;   // 0: Hpsi stale, 1: Hpsi = IFFT_y K0 on all panels, 2: K0 dealiased, Hpsi on retained panels,
+                      // 3: K0 dealiased, Hpsi on the retained panels of the single-field shape svar (layout CfgE<N, svar>)
     int kv_valid = 0;
     int have_part = 0;
     int np_cap = 0;     // particle buffers are sized for this many particles
@@ -158,7 +159,10 @@ struct vk_gpe {
     int ens_var = 0;
     Geo geo_v{};
     // one-member ensemble table: N <= 1024 runs the same kernels as a
-    // batched ensemble member, so both give bit-identical results
+    // batched ensemble member, so both give bit-identical results; svar is
+    // its transform shape (0 as a default ensemble member, 1: CfgE<N, 1>)
+    int svar = 0;
+    Geo geo_s{};
     vk::EnsReal* e1R = nullptr;
     int* e1map[3] = {};
     int* e1off[3] = {};
@@ -419,7 +423,7 @@ int dealias_k(vk_gpe* h, cplx* K)
 
 int ensure_any(vk_gpe* h)
 {
-    if (h->hstate == 0) {
+    if (h->hstate == 0 || h->hstate == 3) {
         int rc = col_inv(h, h->K0, h->Hpsi, 0, 0);
         if (rc) return rc;
         h->hstate = 1;
@@ -587,6 +591,7 @@ void join_particles(vk_gpe* h)
 int ens1_step(vk_gpe* h);
 int make_single_table(vk_gpe* h);
 vk::EnsReal ens_entry(const vk_gpe* h);
+int prep_rk4(vk_gpe* h);
 
 int rk4_step(vk_gpe* h)
 {
@@ -1144,7 +1149,7 @@ int vk_gpe_get_psi(vk_gpe* h, double* psi, double* kpsi)
 int vk_gpe_rk4_steps_async(vk_gpe* h, int nsteps)
 {
     if (!h || nsteps < 0) return fail(VK_ERR_ARG, "rk4_steps: bad arguments");
-    int rc = ensure_rk4(h);
+    int rc = prep_rk4(h);
     if (rc) return rc;
     for (int i = 0; i < nsteps; ++i)
         if ((rc = rk4_step(h))) return rc;
@@ -1564,7 +1569,7 @@ const char* vk_pass_name(int kind) { return (kind >= 0 && kind < PK_COUNT) ? kPa
 int vk_gpe_pass_times(vk_gpe* h, int nsteps, double* ms_out, int nkinds)
 {
     if (!h || nsteps <= 0 || !ms_out) return fail(VK_ERR_ARG, "pass_times: bad arguments");
-    int rc = ensure_rk4(h);
+    int rc = prep_rk4(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     h->timing = true;
@@ -1845,18 +1850,56 @@ vk::EnsReal ens_entry(const vk_gpe* h)
 int make_single_table(vk_gpe* h)
 {
     if (h->N < 128 || h->N > 1024) return 0;
+    // the table and block maps of the single-field shape svar (build_maps
+    // and ens_entry read the member shape fields, borrowed here)
+    const int keep_var = h->ens_var;
+    const Geo keep_geo = h->geo_v;
+    h->ens_var = h->svar;
+    h->geo_v = h->geo_s;
     const vk::EnsReal x = ens_entry(h);
-    HIPCHK(hipMalloc(&h->e1R, sizeof(vk::EnsReal)));
-    HIPCHK(hipMemcpy(h->e1R, &x, sizeof(x), hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; ++k) {
+        if (h->e1map[k]) { hipFree(h->e1map[k]); h->e1map[k] = nullptr; }
+        if (h->e1off[k]) { hipFree(h->e1off[k]); h->e1off[k] = nullptr; }
+    }
     std::vector<vk_gpe*> hs{h};
-    int rc = build_maps(hs, std::vector<int>{0}, h->e1map, h->e1off, h->e1nblk);
+    int rc = 0;
+    if (!h->e1R && hipMalloc(&h->e1R, sizeof(vk::EnsReal)) != hipSuccess) rc = fail(VK_ERR_HIP, "hipMalloc e1R");
+    if (!rc && hipMemcpy(h->e1R, &x, sizeof(x), hipMemcpyHostToDevice) != hipSuccess) rc = fail(VK_ERR_HIP, "e1R copy");
+    if (!rc) rc = build_maps(hs, std::vector<int>{0}, h->e1map, h->e1off, h->e1nblk);
+    h->ens_var = keep_var;
+    h->geo_v = keep_geo;
     if (rc) return rc;
-    dispatch(h->N, [&](auto n) { ens_lds<CfgN<decltype(n)::value>>(h->e1lds); return 0; });
+    for (auto& l : h->e1lds) l = 0;
+    dispatch_var(h->N, h->svar, [&](auto c) { ens_lds<decltype(c)>(h->e1lds); return 0; });
     set_lds(vk::k_ens_p1, h->e1lds[0]);
     set_lds(vk::k_ens_p2, h->e1lds[1]);
     set_lds(vk::k_ens_p3, h->e1lds[2]);
     set_lds(vk::k_ens_p4, h->e1lds[3]);
     for (auto k : {vk::k_ens_p4s<1>, vk::k_ens_p4s<2>, vk::k_ens_p4s<3>, vk::k_ens_p4s<4>}) set_lds(k, h->e1lds[3]);
+    return 0;
+}
+
+// Before RK4 steps: K0 dealiased and Hpsi = IFFT_y(K0) on the retained
+// panels in the layout the steps use: the single-field shape's for the
+// one-member ensemble path (hstate 3 with svar = 1), else Cfg<N>'s
+// (hstate 2).
+int prep_rk4(vk_gpe* h)
+{
+    const bool e1 = h->e1R && !(h->have_part && h->p.np > 0);
+    if (!(e1 && h->svar)) return ensure_rk4(h);
+    if (h->hstate == 3) return 0;
+    int rc = ensure_rk4(h);
+    if (rc) return rc;
+    rc = dispatch_var(h->N, h->svar, [&](auto c) {
+        using C = decltype(c);
+        const size_t lds = (size_t)C::TC * C::COLRS * sizeof(double);
+        set_lds(vk::k_col_inv<C>, lds);
+        hipLaunchKernelGGL(vk::k_col_inv<C>, dim3(h->geo_s.Pp + h->geo_s.Pn), dim3(C::COLT), lds, h->stream,
+                           (const cplx*)h->K0, h->Hpsi, (const cplx*)h->tw, h->geo_s, 1, 0);
+        return 0;
+    });
+    if (rc) return rc;
+    h->hstate = 3;
     return 0;
 }
 
@@ -2610,6 +2653,33 @@ int vk_ens_autotune_shapes(vk_ens* e, int nsteps, int* vars_out, double* ms_out)
     hipEventDestroy(a);
     hipEventDestroy(b);
     return rc;
+}
+
+}  // extern "C"
+
+// single-field transform shape (defined here, after the ensemble helpers it uses)
+extern "C" {
+
+int vk_gpe_set_shape(vk_gpe* h, int var)
+{
+    if (!h || (var != 0 && var != 1)) return fail(VK_ERR_ARG, "set_shape: bad arguments");
+    if (!h->e1R || !vk::has_variant(h->N, var) || h->p.np > 0)
+        return fail(VK_ERR_UNSUPPORTED, "set_shape: shapes exist for N in {256, 512, 1024} without particles");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (var == h->svar) return 0;
+    if (h->hstate == 3) h->hstate = 0;   // Hpsi in the old shape's layout
+    h->svar = var;
+    int TC = 0;
+    dispatch_var(h->N, var, [&](auto c) { TC = decltype(c)::TC; return 0; });
+    h->geo_s = var ? geo_of(h->N, TC) : h->geo;
+    return make_single_table(h);
+}
+
+int vk_gpe_get_shape(vk_gpe* h, int* var)
+{
+    if (!h || !var) return fail(VK_ERR_ARG, "get_shape: bad arguments");
+    *var = h->svar;
+    return 0;
 }
 
 }  // extern "C"

@@ -285,6 +285,16 @@ class GPE:
         names = [nat.lib().vk_pass_name(i).decode() for i in range(k)]
         return dict(zip(names, ms.tolist()))
 
+    def set_shape(self, var):
+        """Transform shape of the RK4 step of a field of N in {256, 512,
+        1024} without particles: 0 (default) or 1 (8 points per thread)."""
+        nat.check(nat.lib().vk_gpe_set_shape(self.h, int(var)))
+
+    def shape(self):
+        v = nat.ctypes.c_int()
+        nat.check(nat.lib().vk_gpe_get_shape(self.h, nat.ctypes.byref(v)))
+        return v.value
+
     def set_row_variants(self, p1, p3):
         """Row-pass variants of the psi-row path (N >= 2048): p1 0 = TR rows
         per workgroup, 1 = one row, 2 = the prefetching persistent P1;

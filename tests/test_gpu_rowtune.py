@@ -1,4 +1,6 @@
-"""Row-pass variants of the psi-row path (N >= 2048) and their autotuner:
+"""Kernel variants of one field: the row-pass variants of the psi-row path
+(N >= 2048) and their autotuner, and the transform shape of small fields.
+Row variants and tuner:
 the tuner restores the state it started from (kpsi and particle state, bit
 for bit), keeps the fastest variant pair, and a handle set to that pair by
 hand continues exactly as the tuned one; every variant agrees with the
@@ -89,3 +91,52 @@ def test_row_variants_agree(G):
     with pytest.raises(nat.VKError):
         g.set_row_variants(2, 0)             # no prefetching P1 at 2048^2
     g.close()
+
+
+@pytest.mark.parametrize("N", [256, 512, 1024])
+def test_single_field_shape(G, oracle_mod, N):
+    """A single field of N <= 1024 stepped in its second transform shape (8
+    points per thread): within roundoff of the default shape, also with
+    psi reads and diagnostics between steps (which rebuild the y-inverse in
+    the default layout); within 1e-11 of the oracle; the shape can be set
+    back; fields with particles have one shape."""
+    say("case single-field shape 1, N=%d" % N)
+    p = G.params_from_namelist(N, 1e-3, 2.5)
+    xs, ys, qs = G.random_vortices(16, 40 + N % 7)
+    a = G.GPE(p)
+    a.set_ic_vortices(xs, ys, qs)
+    psi0 = a.psi
+    a.gpe_rk4(6)
+    ref = a.psi
+    a.close()
+    b = G.GPE(p)
+    b.set_ic_vortices(xs, ys, qs)
+    b.set_shape(1)
+    assert b.shape() == 1
+    b.gpe_rk4(3)
+    _ = b.psi
+    _ = b.diagnostics_gpe()
+    b.gpe_rk4(3)
+    assert rel_l2(b.psi, ref) < 1e-13
+    c = G.GPE(p)
+    c.set_ic_vortices(xs, ys, qs)
+    c.set_shape(1)
+    c.gpe_rk4(6)
+    assert rel_l2(c.psi, b.psi) < 1e-13
+    c.set_shape(0)
+    c.gpe_rk4(1)
+    b.gpe_rk4(1)
+    assert rel_l2(c.psi, b.psi) < 1e-13
+    b.close()
+    c.close()
+    if N == 256:
+        O = oracle_mod
+        st = O.State(O.make_params(N, 1e-3, 2.5))
+        st.set_psi(psi0, sync=True)
+        st.rk4(6)
+        assert rel_l2(ref, st.psi) < 1e-11
+    from quantumvortices_amd import _native as nat
+    q = G.GPE(G.params_from_namelist(N, 1e-3, 2.5, gp_N_obj=2, ppotV0=10.0, ppotd=2.0, gp_mass_part=5.0))
+    with pytest.raises(nat.VKError):
+        q.set_shape(1)
+    q.close()
