@@ -323,6 +323,34 @@ def diagnostics_bench(G, torch):
     return out
 
 
+def small_field_bench(G, torch):
+    """RK4 of one 1024^2 field (the 1024^2 configuration's size, 64 seeded
+    vortices, dt = 5e-4, no particles) in both transform shapes: ms per
+    step over 200 steps after 20 warm, and the compulsory HBM bytes of a
+    step (the four passes of the four stages) over the faster time."""
+    N = 1024
+    p = G.params_from_namelist(N, 5e-4, 2.5)
+    xs, ys, qs = G.random_vortices(64, 1)
+    out = {"workload": "GPE RK4 1024^2 FP64, dt=5e-4, 64 random-sign vortices (seed 1), one field"}
+    best = None
+    for var in (0, 1):
+        g = G.GPE(p)
+        g.set_ic_vortices(xs, ys, qs, nimg=3)
+        g.set_shape(var)
+        ms = time_steps(torch, g, 200, 20) / 200
+        out["ms_per_step_shape%d" % var] = round(ms, 4)
+        if best is None or ms < best[0]:
+            pb = g.pass_bytes()
+            best = (ms, 4.0 * sum(pb[k] for k in ("p1_rho_row", "p2_dealias_col", "p3_nonlin_row", "p4_rk4_col")))
+        g.close()
+    ms, step_bytes = best
+    out["steps_per_s"] = round(1e3 / ms, 1)
+    ach = step_bytes / (ms * 1e-3) / 1e9
+    out["step_roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach / HBM_PEAK_GBS, 4), "compulsory_bytes_per_step": step_bytes}
+    return out
+
+
 def sts_bench(G, torch):
     """Spatio-temporal spectra at the size of the reference's CompSTS driver
     (128^2, Nt = 2048 snapshots): one snapshot of the retained modes per
@@ -498,9 +526,13 @@ def main():
         log("bench: argle", argle)
 
     diag = None
+    small = None
     if not args.no_diag:
         log("bench: diagnostics at 1024^2 and 4096^2")
         diag = diagnostics_bench(G, torch)
+        log("bench: one 1024^2 field, both transform shapes")
+        small = small_field_bench(G, torch)
+        log("bench: rk4_1024", small)
         log("bench: diagnostics", diag)
 
     sts = None
@@ -555,6 +587,7 @@ def main():
         "argle_warm_start": argle,
         "spatio_temporal_spectra": sts,
         "diagnostics": diag,
+        "rk4_1024": small,
     }
     print(json.dumps(out), flush=True)
 
